@@ -1,0 +1,135 @@
+/*
+ * qtnh_hip.h -- C ABI of the MI355X (gfx950) kernels behind the qtnh local
+ * contraction path.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types.  Element
+ * type is complex128 stored interleaved (re, im), the layout of
+ * std::complex<double> (reference include/core/typedefs.hpp:21).  Tensors are
+ * row-major over their dims; counts are in elements (64-bit).
+ *
+ * Ownership: the caller owns every buffer; the library owns only the context
+ * (device, stream, workspace).  Threading: a context is not thread-safe.  All
+ * compute calls are stream-ordered and asynchronous on the context's stream;
+ * qtnh_d2h and qtnh_sync block.  No call throws; every call returns a
+ * qtnh_status and, on failure, leaves a message in qtnh_ctx_last_error().
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * jjacobx/qtnh).
+ */
+#ifndef QTNH_HIP_H
+#define QTNH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  QTNH_OK = 0,
+  QTNH_EINVAL = 1,       /* bad argument (shape, axis, wire, placement) */
+  QTNH_ENOMEM = 2,       /* device allocation failed or budget exceeded */
+  QTNH_EHIP = 3,         /* HIP runtime error (no device, launch failure) */
+  QTNH_EUNSUPPORTED = 4  /* valid request outside what this build implements */
+} qtnh_status;
+
+/* Placement marker for wired axes (reference include/core/typedefs.hpp:55, X). */
+#define QTNH_X 0xFFFFu
+/* Largest tensor rank accepted (axes of any dims). */
+#define QTNH_MAX_RANK 64
+
+/* precision argument of qtnh_contract_c128 */
+#define QTNH_PREC_FP64 0
+#define QTNH_PREC_SPLIT 1
+
+/* distribution argument of qtnh_fill_c128 */
+#define QTNH_DIST_NORMAL 0
+#define QTNH_DIST_UNIFORM 1
+
+typedef struct qtnh_ctx_s *qtnh_ctx_t;
+
+/* ---- context: replaces the MPI lifecycle of QTNHEnv (reference src/core/env.cpp:9-23) ---- */
+/* stream: the hipStream_t every call of this context is ordered on (NULL: the
+ * device's default stream).  The caller keeps ownership of the stream. */
+qtnh_status qtnh_ctx_create(int device, void *stream, qtnh_ctx_t *out);
+qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx);
+const char *qtnh_status_string(int status);
+const char *qtnh_ctx_last_error(qtnh_ctx_t ctx);
+/* Number of visible HIP devices (0 when none); never fails on a host without GPUs. */
+qtnh_status qtnh_device_count(int *count);
+
+/* ---- device storage: replaces std::vector<tel> loc_els_ (reference include/tensor/dense.hpp:24) ---- */
+qtnh_status qtnh_buf_alloc(qtnh_ctx_t ctx, uint64_t n, void **out);
+qtnh_status qtnh_buf_free(qtnh_ctx_t ctx, void *buf);
+qtnh_status qtnh_h2d(qtnh_ctx_t ctx, void *dst, const void *src, uint64_t n);
+qtnh_status qtnh_d2h(qtnh_ctx_t ctx, void *dst, const void *src, uint64_t n);
+qtnh_status qtnh_d2d(qtnh_ctx_t ctx, void *dst, const void *src, uint64_t n);
+qtnh_status qtnh_sync(qtnh_ctx_t ctx);
+
+/* Seeded on-device fill (no reference counterpart: synthetic inputs of the
+ * benchmarks).  Element g uses SplitMix64 outputs 2g and 2g+1 of `seed`;
+ * dist QTNH_DIST_NORMAL gives re, im ~ N(0,1) by Box-Muller. */
+qtnh_status qtnh_fill_c128(qtnh_ctx_t ctx, void *buf, uint64_t n, uint64_t seed, int dist);
+
+/* ---- output placement: host logic of Tensor::contract (reference src/tensor/contract.cpp:219-289) ---- */
+/* Default placement (contract.cpp:219-248).  wires: nw (t1 axis, t2 axis) pairs. */
+qtnh_status qtnh_default_repls(int r1, int ndis1, int r2, int ndis2, int nw, const uint16_t *wires,
+                               uint16_t *repl1, uint16_t *repl2);
+/* Symmetric in-place placement (contract.cpp:250-289) for a symmetric t2 with
+ * nd2 distributed axes, nl2 local axes and n_dis_in distributed input axes.
+ * *applies is set to 1 (repls written) or 0 (policy does not apply). */
+qtnh_status qtnh_symm_repls(int r1, int nd2, int nl2, int n_dis_in, int nw, const uint16_t *wires,
+                            uint16_t *repl1, uint16_t *repl2, int *applies);
+/* Result dims of a contraction under the given placement; validates that the
+ * placement is a permutation of the result axes (contract.cpp:86-111). */
+qtnh_status qtnh_contract_dims(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw,
+                               const uint16_t *wires, const uint16_t *repl1, const uint16_t *repl2,
+                               int *rC, uint64_t *dC);
+
+/* ---- data movement ---- */
+/* Axis permutation, out of place: dst axis ptup[i] = src axis i.
+ * Replaces DenseTensor::permute / TIDense::_permute_internal (reference
+ * src/tensor/dense.cpp:152-169, 354-488; Tensor::permute tensor.hpp:190-192). */
+qtnh_status qtnh_permute_c128(qtnh_ctx_t ctx, const void *src, void *dst, int rank, const uint64_t *dims,
+                              const uint16_t *ptup);
+/* In-place exchange of two equal-dim axes.  Replaces DenseTensor::swap /
+ * TIDense::_swap_internal local case (reference src/tensor/dense.cpp:108-111, 171-218). */
+qtnh_status qtnh_swap_c128(qtnh_ctx_t ctx, void *buf, int rank, const uint64_t *dims, int i, int j);
+
+/* ---- gate application (the symmetric in-place policy of contract.cpp:250-289 + STEP 3 :131-162) ---- */
+/* In place: psi'[.. o@axes ..] = sum_i psi[.. i@axes ..] * G[i, o], G host
+ * array of prod(d)^2 complex, layout [in_0..in_{k-1}, out_0..out_{k-1}]
+ * (reference include/tensor/symm.hpp:22-45).  Sum over i in row-major order
+ * from 0, products without FMA: the reference's arithmetic, bit for bit.
+ * Replaces Tensor::contract(state, SymmTensor) with all gate inputs wired.
+ * flags: reserved, pass 0. */
+qtnh_status qtnh_gate_apply_c128(qtnh_ctx_t ctx, void *state, int rank, const uint64_t *dims, int k,
+                                 const uint16_t *axes, const double *G, int flags);
+/* In place: psi[x] *= d[x@axes], d host array of prod(d) complex.  The
+ * intended DiagTensor contraction (reference include/tensor/diag.hpp:11-14,
+ * 63-65; src/tensor/diag.cpp:19). */
+qtnh_status qtnh_diag_apply_c128(qtnh_ctx_t ctx, void *state, int rank, const uint64_t *dims, int k,
+                                 const uint16_t *axes, const double *d);
+
+/* ---- general pairwise contraction: _contract_dense (reference src/tensor/contract.cpp:12-201) ---- */
+/* C = sum over wires of A*B, result placed by repl1/repl2 (QTNH_X for wired
+ * axes).  Small problems use a kernel that keeps the reference summation
+ * order; large ones permute to matrices and run a complex GEMM on MFMA.
+ * precision QTNH_PREC_FP64 (the drop-in) or QTNH_PREC_SPLIT; err_out (may
+ * be NULL) receives an a-posteriori relative error estimate for SPLIT and 0
+ * for FP64.  A, B and C must not overlap. */
+qtnh_status qtnh_contract_c128(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B,
+                               int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                               const uint16_t *repl1, const uint16_t *repl2, void *C, int precision,
+                               double *err_out);
+
+/* ---- reductions used for normalisation and checks ---- */
+/* *out = sum |x_i|^2 (device reduction, fixed order for a given n). */
+qtnh_status qtnh_norm2_c128(qtnh_ctx_t ctx, const void *x, uint64_t n, double *out);
+/* x_i *= (re + i im) */
+qtnh_status qtnh_scale_c128(qtnh_ctx_t ctx, void *x, uint64_t n, double re, double im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,90 @@
+"""ctypes binding of libqtnh_hip.so (the C ABI declared in include/qtnh_hip.h).
+
+The library is the only compute path: if it is missing, or there is no GPU,
+every compute call raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libqtnh_hip.so")
+
+QTNH_OK, QTNH_EINVAL, QTNH_ENOMEM, QTNH_EHIP, QTNH_EUNSUPPORTED = range(5)
+X = 0xFFFF
+PREC_FP64, PREC_SPLIT = 0, 1
+DIST_NORMAL, DIST_UNIFORM = 0, 1
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes; every function returns int (qtnh_status) unless listed in _RESTYPE.
+SIGNATURES = {
+    "qtnh_ctx_create": [_i, _p, ctypes.POINTER(_p)],
+    "qtnh_ctx_destroy": [_p],
+    "qtnh_status_string": [_i],
+    "qtnh_ctx_last_error": [_p],
+    "qtnh_device_count": [ctypes.POINTER(_i)],
+    "qtnh_buf_alloc": [_p, _u64, ctypes.POINTER(_p)],
+    "qtnh_buf_free": [_p, _p],
+    "qtnh_h2d": [_p, _p, _p, _u64],
+    "qtnh_d2h": [_p, _p, _p, _u64],
+    "qtnh_d2d": [_p, _p, _p, _u64],
+    "qtnh_sync": [_p],
+    "qtnh_fill_c128": [_p, _p, _u64, _u64, _i],
+    "qtnh_default_repls": [_i, _i, _i, _i, _i, _u16p, _u16p, _u16p],
+    "qtnh_symm_repls": [_i, _i, _i, _i, _i, _u16p, _u16p, _u16p, ctypes.POINTER(_i)],
+    "qtnh_contract_dims": [_i, _u64p, _i, _u64p, _i, _u16p, _u16p, _u16p, ctypes.POINTER(_i), _u64p],
+    "qtnh_permute_c128": [_p, _p, _p, _i, _u64p, _u16p],
+    "qtnh_swap_c128": [_p, _p, _i, _u64p, _i, _i],
+    "qtnh_gate_apply_c128": [_p, _p, _i, _u64p, _i, _u16p, _dp, _i],
+    "qtnh_diag_apply_c128": [_p, _p, _i, _u64p, _i, _u16p, _dp],
+    "qtnh_contract_c128": [_p, _p, _i, _u64p, _p, _i, _u64p, _i, _u16p, _u16p, _u16p, _p, _i, _dp],
+    "qtnh_norm2_c128": [_p, _p, _u64, _dp],
+    "qtnh_scale_c128": [_p, _p, _u64, ctypes.c_double, ctypes.c_double],
+    "qtnh_tune_permute": [_i, _i],
+}
+_RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
+            "qtnh_tune_permute": None}
+
+_LIB = None
+
+
+class QtnhError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("qtnh status %d (%s): %s" % (status, _STATUS_NAMES.get(status, "?"), msg))
+        self.status = status
+
+
+_STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP runtime error",
+                 4: "unsupported"}
+
+
+def lib():
+    """Load libqtnh_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqtnh_hip.so not found at %s; build it with `python -m qtnh_amd.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, ctypes.c_int)
+        _LIB = L
+    return _LIB
+
+
+def check(status, ctx=None):
+    if status != QTNH_OK:
+        msg = lib().qtnh_ctx_last_error(ctx).decode() if ctx else ""
+        raise QtnhError(status, msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().qtnh_device_count(ctypes.byref(n)))
+    return n.value

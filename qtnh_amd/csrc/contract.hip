@@ -1,0 +1,317 @@
+// General pairwise contraction (reference src/tensor/contract.cpp:12-201).
+//
+// Two routes:
+//  * direct: one thread per result element, the closed tuples summed in the
+//    reference order (wires sorted by their t2 axis, last one fastest,
+//    accumulator from 0, products without FMA; contract.cpp:26, 131-156 and
+//    indexing.cpp:27-35).  Used when the contraction is small or one operand
+//    is gate-sized; it reproduces the reference's bits.
+//  * GEMM: permute A to [open1, closed] and B to [closed, open2], complex GEMM
+//    on MFMA, then permute the natural result [open1, open2] to the
+//    placement.  Used when both sides are large (within 1e-12 relative).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace qtnh_hip {
+namespace {
+
+constexpr int kDirThreads = 256;
+constexpr int kMaxOut = 48;     // result axes handled by the direct kernels
+constexpr int kMaxClosed = 24;  // closed axes handled by the direct kernels
+
+__device__ inline void cmac(double &ar, double &ai, c128 a, c128 b) {
+  double pr = a.re * b.re - a.im * b.im;
+  double pi = a.re * b.im + a.im * b.re;
+  ar = ar + pr;
+  ai = ai + pi;
+}
+
+struct Closed {
+  uint64_t kd[kMaxClosed];  // dim of closed tuple position w (reference order)
+  uint64_t ka[kMaxClosed];  // A stride
+  uint64_t kb[kMaxClosed];  // B stride
+  int nw;
+};
+
+// Sum over the closed tuples in reference order (odometer, last position fastest).
+__device__ inline c128 closed_sum(const c128 *__restrict__ A, const c128 *__restrict__ B, uint64_t oa, uint64_t ob,
+                                  const Closed &cl) {
+  double ar = 0.0, ai = 0.0;
+  uint32_t idx[kMaxClosed];
+  for (int w = 0; w < cl.nw; ++w) idx[w] = 0;
+  while (true) {
+    cmac(ar, ai, A[oa], B[ob]);
+    int w = cl.nw - 1;
+    for (; w >= 0; --w) {
+      if (++idx[w] < cl.kd[w]) {
+        oa += cl.ka[w];
+        ob += cl.kb[w];
+        break;
+      }
+      oa -= (cl.kd[w] - 1) * cl.ka[w];
+      ob -= (cl.kd[w] - 1) * cl.kb[w];
+      idx[w] = 0;
+    }
+    if (w < 0) break;
+  }
+  c128 r;
+  r.re = ar;
+  r.im = ai;
+  return r;
+}
+
+// Result axes in C memory order: dim, and the A / B stride of the open axis placed there.
+struct OutGen {
+  uint64_t cd[kMaxOut];
+  uint64_t ca[kMaxOut];
+  uint64_t cb[kMaxOut];
+  int rc;
+};
+
+__global__ void __launch_bounds__(kDirThreads) contract_direct_general(const c128 *__restrict__ A,
+                                                                       const c128 *__restrict__ B,
+                                                                       c128 *__restrict__ C, OutGen out, Closed cl,
+                                                                       uint64_t nC) {
+  for (uint64_t e = (uint64_t)blockIdx.x * kDirThreads + threadIdx.x; e < nC; e += (uint64_t)gridDim.x * kDirThreads) {
+    uint64_t rest = e, oa = 0, ob = 0;
+    for (int p = out.rc - 1; p >= 0; --p) {
+      uint64_t d = out.cd[p];
+      uint64_t i = rest % d;
+      rest /= d;
+      oa += i * out.ca[p];
+      ob += i * out.cb[p];
+    }
+    C[e] = closed_sum(A, B, oa, ob, cl);
+  }
+}
+
+// Power-of-two result dims: C index bit p comes from A bit abit[p] or B bit
+// bbit[p] (-1 when from the other side).  A block handles 256 consecutive
+// result elements per step; the low 8 bits are per-thread constants.
+struct OutBits {
+  int8_t abit[64];
+  int8_t bbit[64];
+  int nbits;
+};
+
+__global__ void __launch_bounds__(kDirThreads) contract_direct_bits(const c128 *__restrict__ A,
+                                                                    const c128 *__restrict__ B, c128 *__restrict__ C,
+                                                                    OutBits out, Closed cl, uint64_t nC) {
+  const int lowbits = out.nbits < 8 ? out.nbits : 8;
+  uint64_t la = 0, lb = 0;
+  for (int p = 0; p < lowbits; ++p) {
+    uint64_t bit = (threadIdx.x >> p) & 1u;
+    if (out.abit[p] >= 0) la |= bit << out.abit[p];
+    if (out.bbit[p] >= 0) lb |= bit << out.bbit[p];
+  }
+  const uint64_t nhi = nC >> lowbits;
+  if (threadIdx.x >= (1u << lowbits)) return;
+  for (uint64_t h = blockIdx.x; h < nhi; h += gridDim.x) {
+    uint64_t ha = 0, hb = 0;
+    for (int p = lowbits; p < out.nbits; ++p) {
+      uint64_t bit = (h >> (p - lowbits)) & 1ull;
+      if (out.abit[p] >= 0) ha |= bit << out.abit[p];
+      if (out.bbit[p] >= 0) hb |= bit << out.bbit[p];
+    }
+    C[(h << lowbits) | threadIdx.x] = closed_sum(A, B, ha | la, hb | lb, cl);
+  }
+}
+
+void strides(int r, const uint64_t *d, uint64_t *st) {
+  uint64_t s = 1;
+  for (int i = r - 1; i >= 0; --i) {
+    st[i] = s;
+    s *= d[i];
+  }
+}
+
+// Everything the routes need about one contraction.
+struct Plan {
+  int rA, rB, rC, nw;
+  uint64_t dC[kMaxRank];
+  std::vector<int> open1, open2;   // ascending axis lists
+  std::vector<int> cA, cB;         // closed axes in reference order
+  uint64_t M = 1, N = 1, K = 1, nC = 1;
+  std::vector<int> pos;            // result position of natural axis q
+};
+
+Plan make_plan(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+               const uint16_t *r1, const uint16_t *r2) {
+  Plan P;
+  P.rA = rA;
+  P.rB = rB;
+  P.nw = nw;
+  contract_dims(rA, dA, rB, dB, nw, wires, r1, r2, &P.rC, P.dC);
+  std::vector<int> order(nw);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return wires[2 * x + 1] < wires[2 * y + 1]; });
+  std::vector<char> wa(rA, 0), wb(rB, 0);
+  for (int w : order) {
+    P.cA.push_back(wires[2 * w]);
+    P.cB.push_back(wires[2 * w + 1]);
+    wa[wires[2 * w]] = 1;
+    wb[wires[2 * w + 1]] = 1;
+    P.K *= dA[wires[2 * w]];
+  }
+  for (int i = 0; i < rA; ++i)
+    if (!wa[i]) {
+      P.open1.push_back(i);
+      P.M *= dA[i];
+      P.pos.push_back(r1[i]);
+    }
+  for (int i = 0; i < rB; ++i)
+    if (!wb[i]) {
+      P.open2.push_back(i);
+      P.N *= dB[i];
+      P.pos.push_back(r2[i]);
+    }
+  P.nC = P.M * P.N;
+  return P;
+}
+
+}  // namespace
+
+qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
+                                   const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *r1,
+                                   const uint16_t *r2, c128 *C) {
+  Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
+  if (P.nw > kMaxClosed || P.rC > kMaxOut)
+    return set_error(ctx, QTNH_EUNSUPPORTED, "contract: %d closed / %d result axes exceed the direct kernel", P.nw,
+                     P.rC);
+  uint64_t stA[kMaxRank], stB[kMaxRank];
+  strides(rA, dA, stA);
+  strides(rB, dB, stB);
+  Closed cl;
+  memset(&cl, 0, sizeof(cl));
+  cl.nw = P.nw;
+  for (int w = 0; w < P.nw; ++w) {
+    cl.kd[w] = dA[P.cA[w]];
+    cl.ka[w] = stA[P.cA[w]];
+    cl.kb[w] = stB[P.cB[w]];
+  }
+  if (P.K == 0) {
+    // Empty sums: the result is zero.
+    return check_hip(ctx, hipMemsetAsync(C, 0, P.nC * sizeof(c128), ctx->stream), "contract zero fill");
+  }
+  // Natural axis q -> source stride (A or B); result axis p holds natural axis inv[p].
+  const int n1 = (int)P.open1.size();
+  std::vector<int> inv(P.rC);
+  for (int q = 0; q < P.rC; ++q) inv[P.pos[q]] = q;
+  BitLayout LA = bit_layout(rA, dA), LB = bit_layout(rB, dB), LC = bit_layout(P.rC, P.dC);
+  if (LA.binary && LB.binary && LC.binary) {
+    OutBits ob;
+    memset(&ob, -1, sizeof(ob));
+    ob.nbits = LC.nbits;
+    for (int p = 0; p < P.rC; ++p) {
+      int q = inv[p];
+      for (int t = 0; t < LC.width[p]; ++t) {
+        int cbit = LC.shift[p] + t;
+        if (q < n1)
+          ob.abit[cbit] = (int8_t)(LA.shift[P.open1[q]] + t);
+        else
+          ob.bbit[cbit] = (int8_t)(LB.shift[P.open2[q - n1]] + t);
+      }
+    }
+    const int lowbits = std::min(LC.nbits, 8);
+    const uint64_t nhi = P.nC >> lowbits;
+    unsigned blocks = (unsigned)std::min<uint64_t>(nhi, 1u << 20);
+    hipLaunchKernelGGL(contract_direct_bits, dim3(blocks), dim3(kDirThreads), 0, ctx->stream, A, B, C, ob, cl, P.nC);
+    return check_hip(ctx, hipGetLastError(), "contract_direct_bits launch");
+  }
+  OutGen og;
+  memset(&og, 0, sizeof(og));
+  og.rc = P.rC;
+  for (int p = 0; p < P.rC; ++p) {
+    int q = inv[p];
+    og.cd[p] = P.dC[p];
+    if (q < n1)
+      og.ca[p] = stA[P.open1[q]];
+    else
+      og.cb[p] = stB[P.open2[q - n1]];
+  }
+  unsigned blocks = grid_for(P.nC, kDirThreads, 1u << 20);
+  hipLaunchKernelGGL(contract_direct_general, dim3(blocks), dim3(kDirThreads), 0, ctx->stream, A, B, C, og, cl, P.nC);
+  return check_hip(ctx, hipGetLastError(), "contract_direct_general launch");
+}
+
+// Route choice: the direct kernel re-reads each operand element once per
+// result element that uses it, which is cheap while the closed size or one
+// open side is small (gates, vectors, tiny tensors).
+static bool use_direct(const Plan &P) {
+  if (P.nw > kMaxClosed || P.rC > kMaxOut) return false;
+  if (P.K <= 16 || P.M <= 4 || P.N <= 4) return true;
+  return P.M * P.N * P.K <= (1ull << 18);
+}
+
+qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
+                            const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *r1, const uint16_t *r2,
+                            c128 *C, int precision, double *err_out) {
+  Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
+  if (precision == QTNH_PREC_SPLIT)
+    return set_error(ctx, QTNH_EUNSUPPORTED, "contract: split precision is not built yet");
+  if (use_direct(P)) return launch_contract_direct(ctx, A, rA, dA, B, rB, dB, nw, wires, r1, r2, C);
+
+  // GEMM route.  A' = A permuted to [open1, closed]; B' = B permuted to [closed, open2].
+  std::vector<uint16_t> pA(rA), pB(rB), pC(P.rC);
+  int q = 0;
+  for (int i : P.open1) pA[i] = (uint16_t)q++;
+  for (int i : P.cA) pA[i] = (uint16_t)q++;
+  q = 0;
+  for (int i : P.cB) pB[i] = (uint16_t)q++;
+  for (int i : P.open2) pB[i] = (uint16_t)q++;
+  bool idA = true, idB = true, idC = true;
+  for (int i = 0; i < rA; ++i) idA = idA && pA[i] == i;
+  for (int i = 0; i < rB; ++i) idB = idB && pB[i] == i;
+  for (int i = 0; i < P.rC; ++i) {
+    pC[i] = (uint16_t)P.pos[i];
+    idC = idC && P.pos[i] == i;
+  }
+  const uint64_t nA = P.M * P.K, nB = P.K * P.N;
+  auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
+  const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC));
+  char *ws = nullptr;
+  if (bytes) {
+    void *w;
+    qtnh_status st = workspace(ctx, bytes, &w);
+    if (st) return st;
+    ws = (char *)w;
+  }
+  const c128 *Ap = A, *Bp = B;
+  c128 *Cn = C;
+  uint64_t off = 0;
+  if (!idA) {
+    c128 *t = (c128 *)(ws + off);
+    off += pad(nA);
+    qtnh_status st = launch_permute(ctx, A, t, rA, dA, pA.data());
+    if (st) return st;
+    Ap = t;
+  }
+  if (!idB) {
+    c128 *t = (c128 *)(ws + off);
+    off += pad(nB);
+    qtnh_status st = launch_permute(ctx, B, t, rB, dB, pB.data());
+    if (st) return st;
+    Bp = t;
+  }
+  if (!idC) {
+    Cn = (c128 *)(ws + off);
+    off += pad(P.nC);
+  }
+  qtnh_status st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N);
+  if (st) return st;
+  if (!idC) {
+    // Natural result dims: open1 dims then open2 dims.
+    std::vector<uint64_t> dn;
+    for (int i : P.open1) dn.push_back(dA[i]);
+    for (int i : P.open2) dn.push_back(dB[i]);
+    st = launch_permute(ctx, Cn, C, P.rC, dn.data(), pC.data());
+    if (st) return st;
+  }
+  if (err_out) *err_out = 0.0;
+  return QTNH_OK;
+}
+
+}  // namespace qtnh_hip

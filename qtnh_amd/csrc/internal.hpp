@@ -1,0 +1,92 @@
+// Internal declarations shared by the HIP translation units of libqtnh_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "qtnh_hip.h"
+
+struct qtnh_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cus = 0;
+  std::string last_error;
+  // Growable device workspace for permute-to-matrix temporaries.
+  void *ws = nullptr;
+  uint64_t ws_bytes = 0;
+  // Small pinned host buffer + device buffer for reduction results.
+  double *host_scalar = nullptr;
+  double *dev_scalar = nullptr;
+};
+
+namespace qtnh_hip {
+
+// Complex element as stored in HBM: 16 bytes, (re, im).
+struct c128 {
+  double re, im;
+};
+
+constexpr int kMaxRank = QTNH_MAX_RANK;
+
+qtnh_status set_error(qtnh_ctx_t ctx, qtnh_status st, const char *fmt, ...);
+qtnh_status check_hip(qtnh_ctx_t ctx, hipError_t e, const char *what);
+// Ensures ctx->ws holds at least `bytes`; returns its pointer in *out.
+qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out);
+
+// Row-major size with overflow detection; false on overflow.
+bool checked_size(int rank, const uint64_t *dims, uint64_t *n);
+
+// Binary view of a tensor whose dims are all powers of two: axis i occupies
+// index bits [shift[i], shift[i] + width[i]) (row-major: later axes lower).
+struct BitLayout {
+  bool binary = false;  // every dim is a power of two
+  int nbits = 0;
+  int shift[kMaxRank];
+  int width[kMaxRank];
+};
+BitLayout bit_layout(int rank, const uint64_t *dims);
+
+// ---- kernels (launch wrappers; argument checks happen in the ABI layer) ----
+qtnh_status launch_fill(qtnh_ctx_t ctx, c128 *buf, uint64_t n, uint64_t seed, int dist);
+qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
+                           const uint16_t *ptup);
+qtnh_status launch_swap(qtnh_ctx_t ctx, c128 *buf, int rank, const uint64_t *dims, int i, int j);
+qtnh_status launch_gate(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k,
+                        const uint16_t *axes, const double *G);
+qtnh_status launch_diag(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k,
+                        const uint16_t *axes, const double *d);
+qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
+                            const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
+                            const uint16_t *repl2, c128 *C, int precision, double *err_out);
+// Direct kernels only (reference summation order, no workspace use).
+qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
+                                   const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
+                                   const uint16_t *repl2, c128 *C);
+qtnh_status launch_norm2(qtnh_ctx_t ctx, const c128 *x, uint64_t n, double *out);
+qtnh_status launch_scale(qtnh_ctx_t ctx, c128 *x, uint64_t n, double re, double im);
+
+// Complex GEMM C[MxN] = A[MxK] * B[KxN], row-major with leading dims lda/ldb/ldc (elements).
+qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N,
+                             uint64_t K, uint64_t lda, uint64_t ldb, uint64_t ldc);
+
+// Host helpers for placement (also exported through the ABI).
+qtnh_status default_repls(int r1, int ndis1, int r2, int ndis2, int nw, const uint16_t *wires, uint16_t *repl1,
+                          uint16_t *repl2);
+qtnh_status symm_repls(int r1, int nd2, int nl2, int n_dis_in, int nw, const uint16_t *wires, uint16_t *repl1,
+                       uint16_t *repl2, int *applies);
+qtnh_status contract_dims(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                          const uint16_t *repl1, const uint16_t *repl2, int *rC, uint64_t *dC);
+
+// Grid helper: at most `cap` blocks of `threads`, enough for `work` items.
+inline unsigned grid_for(uint64_t work, unsigned threads, unsigned cap) {
+  uint64_t b = (work + threads - 1) / threads;
+  if (b == 0) b = 1;
+  return (unsigned)(b < cap ? b : cap);
+}
+
+}  // namespace qtnh_hip

@@ -1,0 +1,291 @@
+// Axis permutation and axis swap for complex128 tensors.
+//
+// Semantics (reference src/tensor/dense.cpp:152-169, 354-488): the result's
+// axis ptup[i] is the source's axis i.  Pure data movement, bit-exact.
+//
+// Power-of-two dims: the permutation is a permutation of index bits.  A
+// workgroup moves one tile = the 2^a lowest source elements (contiguous in the
+// source) x the source bits that land in the 2^b lowest result elements
+// (contiguous in the result), staged through LDS, so that both the global
+// reads and the global writes are runs of >= 2^min(a,b) * 16 bytes.  The LDS
+// slot of tile element t is t ^ ((t >> a) & (2^a - 1)): reads in source order
+// and writes in result order both spread over the banks.
+//
+// Other dims: one thread per result element with mixed-radix decoding.
+#include <algorithm>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace qtnh_hip {
+namespace {
+
+constexpr int kPermThreads = 256;
+constexpr int kMaxEpt = 16;  // tile elements per thread at 2^12 elements / 256 threads
+
+struct PermBitsArgs {
+  uint64_t in_hi[64];   // source offset of the tile bits above a, indexed by t >> a
+  uint64_t out_hi[64];  // result offset of the tile bits above b, indexed by s >> b
+  uint16_t s2t_lo[64];  // tile index t of result-order index s, from s bits [0, 6)
+  uint16_t s2t_hi[64];  //   ... and from s bits [6, 12)
+  uint8_t ob_in[64];    // source bit of outer bit j
+  uint8_t ob_out[64];   // result bit of outer bit j
+  int a, b, u;          // source-low bits, result-low bits, tile bits
+  int n_outer;
+};
+
+__device__ inline uint32_t lds_slot(uint32_t t, int a) { return t ^ ((t >> a) & ((1u << a) - 1u)); }
+
+template <typename Off>
+__global__ void __launch_bounds__(kPermThreads) permute_bits_kernel(const c128 *__restrict__ src,
+                                                                    c128 *__restrict__ dst, PermBitsArgs args,
+                                                                    uint64_t n_tiles) {
+  extern __shared__ c128 tile[];
+  const uint32_t T = 1u << args.u;
+  const uint32_t amask = (1u << args.a) - 1u;
+  const uint32_t bmask = (1u << args.b) - 1u;
+  const int ept = (int)((T + kPermThreads - 1) / kPermThreads);
+  // Per-thread tile offsets are the same for every tile: compute them once.
+  Off xo[kMaxEpt], yo[kMaxEpt];
+  uint16_t ls[kMaxEpt], lt[kMaxEpt];
+#pragma unroll
+  for (int r = 0; r < kMaxEpt; ++r) {
+    uint32_t e = threadIdx.x + r * kPermThreads;
+    if (r < ept && e < T) {
+      xo[r] = (Off)((e & amask) | args.in_hi[e >> args.a]);
+      ls[r] = (uint16_t)lds_slot(e, args.a);
+      uint32_t t = args.s2t_lo[e & 63u] | args.s2t_hi[e >> 6];
+      yo[r] = (Off)((e & bmask) | args.out_hi[e >> args.b]);
+      lt[r] = (uint16_t)lds_slot(t, args.a);
+    } else {
+      xo[r] = 0;
+      yo[r] = 0;
+      ls[r] = 0;
+      lt[r] = 0;
+    }
+  }
+  for (uint64_t g = blockIdx.x; g < n_tiles; g += gridDim.x) {
+    uint64_t xb = 0, yb = 0;
+    for (int j = 0; j < args.n_outer; ++j) {
+      uint64_t bit = (g >> j) & 1ull;
+      xb |= bit << args.ob_in[j];
+      yb |= bit << args.ob_out[j];
+    }
+    const c128 *s = src + xb;
+    c128 *d = dst + yb;
+    c128 v[kMaxEpt];
+#pragma unroll
+    for (int r = 0; r < kMaxEpt; ++r)
+      if (r < ept && threadIdx.x + r * kPermThreads < T) v[r] = s[xo[r]];
+#pragma unroll
+    for (int r = 0; r < kMaxEpt; ++r)
+      if (r < ept && threadIdx.x + r * kPermThreads < T) tile[ls[r]] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kMaxEpt; ++r)
+      if (r < ept && threadIdx.x + r * kPermThreads < T) d[yo[r]] = tile[lt[r]];
+    __syncthreads();
+  }
+}
+
+// General dims: result element y -> source offset.  nd: result dims, jump:
+// source stride of the source axis placed at each result axis.
+struct PermGenArgs {
+  uint64_t nd[kMaxRank];
+  uint64_t jump[kMaxRank];
+  int rank;
+};
+
+__global__ void permute_general_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, PermGenArgs args,
+                                       uint64_t n) {
+  for (uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; y < n; y += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t rest = y, x = 0;
+    for (int q = args.rank - 1; q >= 0; --q) {
+      uint64_t d = args.nd[q];
+      uint64_t i = rest % d;
+      rest /= d;
+      x += i * args.jump[q];
+    }
+    dst[y] = src[x];
+  }
+}
+
+__global__ void copy_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// Swap of two single-bit axes at bit positions lo < hi: exchange the elements
+// with (bit hi, bit lo) = (1, 0) and (0, 1).  One thread per such pair.
+__global__ void swap_bits_kernel(c128 *__restrict__ buf, int lo, int hi, uint64_t n_pairs) {
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_pairs;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t b = g;
+    b = ((b >> lo) << (lo + 1)) | (b & ((1ull << lo) - 1));
+    b = ((b >> hi) << (hi + 1)) | (b & ((1ull << hi) - 1));
+    uint64_t e1 = b | (1ull << hi), e2 = b | (1ull << lo);
+    c128 v1 = buf[e1], v2 = buf[e2];
+    buf[e1] = v2;
+    buf[e2] = v1;
+  }
+}
+
+// Swap of axes with strides si, sj and common dim d: the thread of the
+// element whose value at axis i is below its value at axis j exchanges it
+// with the transposed element.
+__global__ void swap_general_kernel(c128 *__restrict__ buf, uint64_t n, uint64_t si, uint64_t sj, uint64_t d) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t vi = (e / si) % d, vj = (e / sj) % d;
+    if (vi < vj) {
+      uint64_t o = e + (vj - vi) * si - (vj - vi) * sj;
+      c128 v1 = buf[e], v2 = buf[o];
+      buf[e] = v2;
+      buf[o] = v1;
+    }
+  }
+}
+
+}  // namespace
+
+// Tunables for the bit-permute tile (source-low bits, result-low bits).
+static int g_perm_a = 6, g_perm_b = 6;
+
+extern "C" void qtnh_tune_permute(int a, int b) {
+  g_perm_a = std::max(1, std::min(a, 6));
+  g_perm_b = std::max(1, std::min(b, 6));
+}
+
+static qtnh_status launch_copy(qtnh_ctx_t ctx, const c128 *src, c128 *dst, uint64_t n) {
+  unsigned blocks = grid_for(n, 256, 65536);
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src, dst, n);
+  return check_hip(ctx, hipGetLastError(), "copy launch");
+}
+
+qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
+                           const uint16_t *ptup) {
+  uint64_t n;
+  checked_size(rank, dims, &n);
+  bool identity = true;
+  for (int i = 0; i < rank; ++i) identity = identity && (ptup[i] == i);
+  // Result dims and strides.
+  uint64_t nd[kMaxRank];
+  for (int i = 0; i < rank; ++i) nd[ptup[i]] = dims[i];
+  if (identity) return launch_copy(ctx, src, dst, n);
+
+  BitLayout in = bit_layout(rank, dims);
+  BitLayout out = bit_layout(rank, nd);
+  if (in.binary && out.binary) {
+    const int nb = in.nbits;
+    // sigma: source bit -> result bit.
+    int sigma[64], inv[64];
+    for (int i = 0; i < rank; ++i)
+      for (int t = 0; t < in.width[i]; ++t) sigma[in.shift[i] + t] = out.shift[ptup[i]] + t;
+    for (int p = 0; p < nb; ++p) inv[sigma[p]] = p;
+    PermBitsArgs args;
+    memset(&args, 0, sizeof(args));
+    int a = std::min(g_perm_a, nb), b = std::min(g_perm_b, nb);
+    // Tile bits: source bits [0, a) then the source bits of result bits [0, b) not already in.
+    int extra[6], ne = 0;
+    for (int p = 0; p < b; ++p)
+      if (inv[p] >= a) extra[ne++] = inv[p];  // ordered by result position
+    const int u = a + ne;
+    // t position of each tile source bit
+    int tpos[64];
+    for (int p = 0; p < 64; ++p) tpos[p] = -1;
+    for (int p = 0; p < a; ++p) tpos[p] = p;
+    for (int m = 0; m < ne; ++m) tpos[extra[m]] = a + m;
+    for (int j = 0; j < (1 << ne); ++j) {
+      uint64_t off = 0;
+      for (int m = 0; m < ne; ++m)
+        if ((j >> m) & 1) off |= 1ull << extra[m];
+      args.in_hi[j] = off;
+    }
+    // Result-order index s: bits [0, b) = result bits [0, b); above them the
+    // remaining tile bits (source bits in [0, a) whose result bit >= b), by result position.
+    int rest[6], nr = 0;
+    for (int p = b; p < nb; ++p)
+      if (inv[p] < a) rest[nr++] = p;  // result positions
+    if (b + nr != u) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
+    for (int j = 0; j < (1 << nr); ++j) {
+      uint64_t off = 0;
+      for (int m = 0; m < nr; ++m)
+        if ((j >> m) & 1) off |= 1ull << rest[m];
+      args.out_hi[j] = off;
+    }
+    int s_to_tbit[12];
+    for (int p = 0; p < b; ++p) s_to_tbit[p] = tpos[inv[p]];
+    for (int m = 0; m < nr; ++m) s_to_tbit[b + m] = tpos[inv[rest[m]]];
+    for (int j = 0; j < 64; ++j) {
+      uint32_t lo = 0, hi = 0;
+      for (int m = 0; m < 6; ++m) {
+        if (!((j >> m) & 1)) continue;
+        if (m < u) lo |= 1u << s_to_tbit[m];
+        if (m + 6 < u) hi |= 1u << s_to_tbit[m + 6];
+      }
+      args.s2t_lo[j] = (uint16_t)lo;
+      args.s2t_hi[j] = (uint16_t)hi;
+    }
+    int no = 0;
+    for (int p = 0; p < nb; ++p) {
+      if (tpos[p] >= 0) continue;
+      args.ob_in[no] = (uint8_t)p;
+      args.ob_out[no] = (uint8_t)sigma[p];
+      ++no;
+    }
+    args.a = a;
+    args.b = b;
+    args.u = u;
+    args.n_outer = no;
+    if (u > 12 || u < 0 || no + u != nb) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
+    const uint64_t n_tiles = 1ull << no;
+    const size_t lds = (size_t)sizeof(c128) << u;
+    const unsigned cap = (unsigned)std::max(1, ctx->num_cus) * 16;
+    unsigned blocks = (unsigned)std::min<uint64_t>(n_tiles, cap);
+    if (nb <= 32)
+      hipLaunchKernelGGL(permute_bits_kernel<uint32_t>, dim3(blocks), dim3(kPermThreads), lds, ctx->stream, src, dst,
+                         args, n_tiles);
+    else
+      hipLaunchKernelGGL(permute_bits_kernel<uint64_t>, dim3(blocks), dim3(kPermThreads), lds, ctx->stream, src, dst,
+                         args, n_tiles);
+    return check_hip(ctx, hipGetLastError(), "permute_bits launch");
+  }
+
+  PermGenArgs g;
+  memset(&g, 0, sizeof(g));
+  uint64_t st[kMaxRank];
+  uint64_t s = 1;
+  for (int i = rank - 1; i >= 0; --i) {
+    st[i] = s;
+    s *= dims[i];
+  }
+  for (int i = 0; i < rank; ++i) {
+    g.nd[ptup[i]] = dims[i];
+    g.jump[ptup[i]] = st[i];
+  }
+  g.rank = rank;
+  unsigned blocks = grid_for(n, 256, 65536);
+  hipLaunchKernelGGL(permute_general_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src, dst, g, n);
+  return check_hip(ctx, hipGetLastError(), "permute_general launch");
+}
+
+qtnh_status launch_swap(qtnh_ctx_t ctx, c128 *buf, int rank, const uint64_t *dims, int i, int j) {
+  uint64_t n;
+  checked_size(rank, dims, &n);
+  BitLayout L = bit_layout(rank, dims);
+  if (L.binary && dims[i] == 2) {
+    // axis i < j: axis j holds the lower bit
+    int lo = L.shift[j], hi = L.shift[i];
+    uint64_t pairs = n >> 2;
+    unsigned blocks = grid_for(pairs, 256, 65536);
+    hipLaunchKernelGGL(swap_bits_kernel, dim3(blocks), dim3(256), 0, ctx->stream, buf, lo, hi, pairs);
+    return check_hip(ctx, hipGetLastError(), "swap_bits launch");
+  }
+  uint64_t si = 1, sj = 1;
+  for (int k = rank - 1; k > i; --k) si *= dims[k];
+  for (int k = rank - 1; k > j; --k) sj *= dims[k];
+  unsigned blocks = grid_for(n, 256, 65536);
+  hipLaunchKernelGGL(swap_general_kernel, dim3(blocks), dim3(256), 0, ctx->stream, buf, n, si, sj, dims[i]);
+  return check_hip(ctx, hipGetLastError(), "swap_general launch");
+}
+
+}  // namespace qtnh_hip

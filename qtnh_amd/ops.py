@@ -1,0 +1,217 @@
+"""Python front end of the gfx950 kernels, over torch device memory.
+
+Tensors are torch complex128 CUDA (HIP) tensors holding row-major data; shapes
+travel separately as `dims` lists (as in the reference, where a tensor is a
+flat element vector plus its dims).  Every call goes through the C ABI of
+libqtnh_hip.so on the caller's current torch stream; nothing here computes on
+the CPU.
+
+Placement arguments follow reference src/tensor/contract.cpp: repl1/repl2
+give each open axis its position in the result (X for wired axes).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import PREC_FP64, PREC_SPLIT, X, check, lib
+
+_CTX = {}
+
+
+def context(device=None, stream=None):
+    """The qtnh_ctx_t bound to (device, stream); created on first use."""
+    if device is None:
+        device = torch.cuda.current_device()
+    if stream is None:
+        stream = torch.cuda.current_stream(device)
+    key = (device, int(stream.cuda_stream))
+    ctx = _CTX.get(key)
+    if ctx is None:
+        h = ctypes.c_void_p()
+        check(lib().qtnh_ctx_create(device, ctypes.c_void_p(int(stream.cuda_stream)), ctypes.byref(h)))
+        ctx = h
+        _CTX[key] = ctx
+    return ctx
+
+
+def _u64(v):
+    a = np.ascontiguousarray(np.asarray(list(v), dtype=np.uint64))
+    if a.size == 0:
+        a = np.zeros(1, dtype=np.uint64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def _u16(v):
+    a = np.ascontiguousarray(np.asarray(list(v), dtype=np.uint16).reshape(-1))
+    if a.size == 0:
+        a = np.zeros(1, dtype=np.uint16)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+
+
+def _c128_host(v):
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.complex128).reshape(-1))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ptr(t):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.complex128 or not t.is_cuda:
+        raise TypeError("expected a complex128 device tensor")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _size(dims):
+    n = 1
+    for d in dims:
+        n *= int(d)
+    return n
+
+
+def _check_len(t, dims):
+    if t.numel() != _size(dims):
+        raise ValueError("tensor has %d elements, dims %s need %d" % (t.numel(), list(dims), _size(dims)))
+
+
+def empty(n, device=None):
+    return torch.empty(int(n), dtype=torch.complex128, device=device or "cuda")
+
+
+def fill_(x, seed, dist=_lib.DIST_NORMAL):
+    ctx = context(x.device.index)
+    check(lib().qtnh_fill_c128(ctx, _ptr(x), x.numel(), seed, dist), ctx)
+    return x
+
+
+def permute(x, dims, ptup, out=None):
+    """Result axis ptup[i] = axis i of x (reference dense.cpp:152-169)."""
+    _check_len(x, dims)
+    out = empty(x.numel(), x.device) if out is None else out
+    ctx = context(x.device.index)
+    d, dp = _u64(dims)
+    p, pp = _u16(ptup)
+    if len(ptup) != len(dims):
+        raise ValueError("ptup length differs from rank")
+    check(lib().qtnh_permute_c128(ctx, _ptr(x), _ptr(out), len(dims), dp, pp), ctx)
+    return out
+
+
+def permuted_dims(dims, ptup):
+    nd = [0] * len(dims)
+    for i, d in enumerate(dims):
+        nd[ptup[i]] = d
+    return nd
+
+
+def swap_(x, dims, i, j):
+    """In-place exchange of equal-dim axes i and j (reference dense.cpp:171-218)."""
+    _check_len(x, dims)
+    ctx = context(x.device.index)
+    d, dp = _u64(dims)
+    check(lib().qtnh_swap_c128(ctx, _ptr(x), len(dims), dp, i, j), ctx)
+    return x
+
+
+def gate_apply_(x, dims, axes, g):
+    """In place: x'[.. o@axes ..] = sum_i x[.. i@axes ..] g[i, o]."""
+    _check_len(x, dims)
+    ctx = context(x.device.index)
+    d, dp = _u64(dims)
+    a, ap = _u16(axes)
+    gh, gp = _c128_host(g)
+    k = _size(dims[ax] for ax in axes)
+    if gh.size != k * k:
+        raise ValueError("gate has %d elements, expected %d" % (gh.size, k * k))
+    check(lib().qtnh_gate_apply_c128(ctx, _ptr(x), len(dims), dp, len(axes), ap, gp, 0), ctx)
+    return x
+
+
+def diag_apply_(x, dims, axes, dvals):
+    """In place: x[i] *= dvals[i@axes]."""
+    _check_len(x, dims)
+    ctx = context(x.device.index)
+    d, dp = _u64(dims)
+    a, ap = _u16(axes)
+    dh, dhp = _c128_host(dvals)
+    if dh.size != _size(dims[ax] for ax in axes):
+        raise ValueError("diagonal length mismatch")
+    check(lib().qtnh_diag_apply_c128(ctx, _ptr(x), len(dims), dp, len(axes), ap, dhp), ctx)
+    return x
+
+
+def default_repls(r1, ndis1, r2, ndis2, wires):
+    w, wp = _u16([v for pair in wires for v in pair])
+    o1 = np.zeros(max(r1, 1), dtype=np.uint16)
+    o2 = np.zeros(max(r2, 1), dtype=np.uint16)
+    check(lib().qtnh_default_repls(r1, ndis1, r2, ndis2, len(wires), wp,
+                                   o1.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                   o2.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))))
+    return [int(v) for v in o1[:r1]], [int(v) for v in o2[:r2]]
+
+
+def symm_repls(r1, nd2, nl2, n_dis_in, wires):
+    w, wp = _u16([v for pair in wires for v in pair])
+    r2 = nd2 + nl2
+    o1 = np.zeros(max(r1, 1), dtype=np.uint16)
+    o2 = np.zeros(max(r2, 1), dtype=np.uint16)
+    ap = ctypes.c_int(0)
+    check(lib().qtnh_symm_repls(r1, nd2, nl2, n_dis_in, len(wires), wp,
+                                o1.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                o2.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), ctypes.byref(ap)))
+    if not ap.value:
+        return None
+    return [int(v) for v in o1[:r1]], [int(v) for v in o2[:r2]]
+
+
+def contract_dims(dA, dB, wires, repl1, repl2):
+    a, ap = _u64(dA)
+    b, bp = _u64(dB)
+    w, wp = _u16([v for pair in wires for v in pair])
+    r1, r1p = _u16(repl1)
+    r2, r2p = _u16(repl2)
+    rc = ctypes.c_int(0)
+    out = np.zeros(64, dtype=np.uint64)
+    check(lib().qtnh_contract_dims(len(dA), ap, len(dB), bp, len(wires), wp, r1p, r2p, ctypes.byref(rc),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    return [int(v) for v in out[:rc.value]]
+
+
+def contract(A, dA, B, dB, wires, repl1=None, repl2=None, precision=PREC_FP64, out=None):
+    """Pairwise contraction over `wires`; returns (C, dC, err).  Default placement
+    when repl1/repl2 are omitted (reference contract.cpp:219-248)."""
+    _check_len(A, dA)
+    _check_len(B, dB)
+    if repl1 is None or repl2 is None:
+        repl1, repl2 = default_repls(len(dA), 0, len(dB), 0, wires)
+    dC = contract_dims(dA, dB, wires, repl1, repl2)
+    out = empty(_size(dC), A.device) if out is None else out
+    ctx = context(A.device.index)
+    a, ap = _u64(dA)
+    b, bp = _u64(dB)
+    w, wp = _u16([v for pair in wires for v in pair])
+    r1, r1p = _u16(repl1)
+    r2, r2p = _u16(repl2)
+    err = ctypes.c_double(0.0)
+    check(lib().qtnh_contract_c128(ctx, _ptr(A), len(dA), ap, _ptr(B), len(dB), bp, len(wires), wp, r1p, r2p,
+                                   _ptr(out), precision, ctypes.byref(err)), ctx)
+    return out, dC, err.value
+
+
+def norm2(x):
+    ctx = context(x.device.index)
+    v = ctypes.c_double(0.0)
+    check(lib().qtnh_norm2_c128(ctx, _ptr(x), x.numel(), ctypes.byref(v)), ctx)
+    return v.value
+
+
+def scale_(x, c):
+    ctx = context(x.device.index)
+    c = complex(c)
+    check(lib().qtnh_scale_c128(ctx, _ptr(x), x.numel(), c.real, c.imag), ctx)
+    return x
+
+
+def tune_permute(a, b):
+    lib().qtnh_tune_permute(a, b)

@@ -1,0 +1,282 @@
+"""Parity of the gfx950 kernels with the oracle, through the C ABI.
+
+Bit-exact for data movement (permute, swap), seeded fill, gates and diagonal
+gates (the kernels keep the reference's summation order and separate
+rounding), and for small contractions (direct route).  The GEMM route is
+checked at relative error <= 1e-12 (the FP64 target).
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from _fixtures import contraction_cases, iden_tensor, swap_tensor
+from qtnh_amd import _lib, ops
+
+pytestmark = pytest.mark.gpu
+
+X = _lib.X
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rnd(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+# ------------------------------------------------------------------ fill
+@pytest.mark.parametrize("dist", [0, 1])
+def test_fill_matches_oracle_bitwise(dist):
+    n = 100003
+    for seed in (0, 1, 2 ** 63 + 5):
+        x = ops.fill_(ops.empty(n), seed, dist)
+        assert same_bits(host(x), oracle.fill(n, seed, dist)), (seed, dist)
+
+
+# ------------------------------------------------------------------ permute
+def _perm_cases():
+    pyr = random.Random(5)
+    cases = []
+    for rank in list(range(1, 21)) + [20, 21]:
+        dims = [2] * rank
+        p = list(range(rank))
+        pyr.shuffle(p)
+        cases.append((dims, p))
+    for _ in range(12):  # power-of-two dims of several widths
+        rank = pyr.randint(2, 8)
+        dims = [pyr.choice([1, 2, 4, 8, 16]) for _ in range(rank)]
+        p = list(range(rank))
+        pyr.shuffle(p)
+        cases.append((dims, p))
+    for _ in range(12):  # general dims
+        rank = pyr.randint(1, 7)
+        dims = [pyr.choice([1, 2, 3, 5, 6]) for _ in range(rank)]
+        p = list(range(rank))
+        pyr.shuffle(p)
+        cases.append((dims, p))
+    # fixed structures: reversal, rotation, identity, swap of the two lowest / highest
+    for rank in (12, 18):
+        cases.append(([2] * rank, list(range(rank))[::-1]))
+        cases.append(([2] * rank, [(i + 1) % rank for i in range(rank)]))
+        cases.append(([2] * rank, list(range(rank))))
+        cases.append(([2] * rank, list(range(rank - 2)) + [rank - 1, rank - 2]))
+        cases.append(([2] * rank, [1, 0] + list(range(2, rank))))
+    return cases
+
+
+@pytest.mark.parametrize("case", _perm_cases(), ids=lambda c: "%s-%s" % (c[0][:3], c[1][:3]))
+def test_permute_bitexact(case):
+    dims, p = case
+    n = int(np.prod(dims))
+    x = rnd(n, n + len(dims))
+    y = ops.permute(dev(x), dims, p)
+    assert same_bits(host(y), oracle.permute(x, dims, p))
+
+
+def test_permute_roundtrip_rank24():
+    dims = [2] * 24
+    p = list(range(24))
+    random.Random(24).shuffle(p)
+    inv = list(np.argsort(p))
+    x = ops.fill_(ops.empty(1 << 24), 77)
+    y = ops.permute(x, dims, p)
+    z = ops.permute(y, ops.permuted_dims(dims, p), inv)
+    torch.cuda.synchronize()
+    assert torch.equal(torch.view_as_real(x), torch.view_as_real(z))
+
+
+def test_permute_rejects_bad_tuple():
+    x = ops.empty(8)
+    with pytest.raises(_lib.QtnhError):
+        ops.permute(x, [2, 2, 2], [0, 1, 1])
+
+
+# ------------------------------------------------------------------ swap
+@pytest.mark.parametrize("dims,i,j", [([2] * 10, 0, 9), ([2] * 10, 3, 4), ([2] * 12, 11, 0), ([2] * 3, 1, 2),
+                                      ([4, 2, 4, 2], 0, 2), ([3, 2, 3], 0, 2), ([2, 5, 5, 2], 1, 2),
+                                      ([2, 2], 1, 1)])
+def test_swap_bitexact(dims, i, j):
+    n = int(np.prod(dims))
+    x = rnd(n, 3)
+    y = ops.swap_(dev(x), dims, i, j)
+    assert same_bits(host(y), oracle.swap(x, dims, i, j))
+
+
+def test_swap_rejects_unequal_dims():
+    with pytest.raises(_lib.QtnhError):
+        ops.swap_(ops.empty(6), [2, 3], 0, 1)
+
+
+# ------------------------------------------------------------------ gates
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_gate_bitexact_binary(k):
+    pyr = random.Random(k)
+    rank = 14
+    dims = [2] * rank
+    for trial in range(6):
+        axes = pyr.sample(range(rank), k)
+        if trial == 0:
+            axes = list(range(rank - k, rank))  # lowest bits
+        if trial == 1:
+            axes = list(range(k))               # highest bits
+        x = rnd(1 << rank, trial)
+        g = rnd(4 ** k, 100 + trial)
+        y = ops.gate_apply_(dev(x), dims, axes, g)
+        assert same_bits(host(y), oracle.gate_apply(x, dims, axes, g)), (k, axes)
+
+
+@pytest.mark.parametrize("dims,axes", [([4, 2, 4], [2, 0]), ([2, 3, 3], [1, 2]), ([3, 3, 3, 2], [0]),
+                                       ([2] * 10, [1, 5, 7, 8]), ([2] * 8, [0, 2, 4, 6, 7])])
+def test_gate_bitexact_general(dims, axes):
+    n = int(np.prod(dims))
+    K = int(np.prod([dims[a] for a in axes]))
+    x = rnd(n, 9)
+    g = rnd(K * K, 10)
+    y = ops.gate_apply_(dev(x), dims, axes, g)
+    assert same_bits(host(y), oracle.gate_apply(x, dims, axes, g))
+
+
+@pytest.mark.parametrize("dims,axes", [([2] * 12, [3]), ([2] * 12, [11, 0]), ([2] * 12, [5, 6, 1]),
+                                       ([2] * 12, [0, 1, 2, 3, 4, 5]), ([3, 2, 3], [0, 2]), ([4, 2, 2], [0, 2])])
+def test_diag_bitexact(dims, axes):
+    n = int(np.prod(dims))
+    K = int(np.prod([dims[a] for a in axes]))
+    x = rnd(n, 4)
+    d = np.exp(1j * np.random.default_rng(1).uniform(0, 2 * np.pi, K))
+    y = ops.diag_apply_(dev(x), dims, axes, d)
+    assert same_bits(host(y), oracle.diag_apply(x, dims, axes, d))
+
+
+def test_gate_unitary_roundtrip_rank22():
+    dims = [2] * 22
+    x = ops.fill_(ops.empty(1 << 22), 5)
+    x0 = x.clone()
+    q, _ = np.linalg.qr(rnd(16, 8).reshape(4, 4))
+    g = q.T.reshape(-1)                   # [in, out] = q^T: applies q
+    ops.gate_apply_(x, dims, [3, 17], g)
+    ops.gate_apply_(x, dims, [3, 17], q.conj().reshape(-1))  # applies q^dagger
+    torch.cuda.synchronize()
+    err = (x - x0).norm().item() / x0.norm().item()
+    assert err < 1e-14
+
+
+# ------------------------------------------------------------------ contraction: golden vectors
+def _check_case(t1, d1, t2, d2, t3, d3, wires, repls=None):
+    out, dc, _ = ops.contract(dev(t1), d1, dev(t2), d2, wires, *(repls or (None, None)))
+    got = host(out)
+    assert dc == d3
+    if repls is None:
+        repls = oracle.default_repls(len(d1), 0, len(d2), 0, wires)
+    ref, _ = oracle.contract(t1, d1, t2, d2, wires, *repls)
+    assert same_bits(got, ref)
+    np.testing.assert_allclose(got, t3, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", list(contraction_cases("dense_vals")), ids=lambda c: c[0])
+def test_contract_dense_vals(case):
+    name, t1, d1, t2, d2, t3, d3, wires = case
+    _check_case(t1, d1, t2, d2, t3, d3, wires)
+
+
+@pytest.mark.parametrize("case", list(contraction_cases("swap_vals")), ids=lambda c: c[0])
+def test_contract_swap_vals(case):
+    name, t1, d1, _, d2, t3, d3, wires = case
+    n = d2[0]
+    _check_case(t1, d1, swap_tensor(n), [n] * 4, t3, d3, wires, ops.symm_repls(len(d1), 0, 4, 0, wires))
+
+
+@pytest.mark.parametrize("case", list(contraction_cases("id_vals")), ids=lambda c: c[0])
+def test_contract_id_vals(case):
+    name, t1, d1, _, d2, t3, d3, wires = case
+    k = len(d2) // 2
+    _check_case(t1, d1, iden_tensor(d2[:k]), d2, t3, d3, wires, ops.symm_repls(len(d1), 0, len(d2), 0, wires))
+
+
+@pytest.mark.parametrize("group,nd1,nd2", [("mpi2r_vals", 1, 0), ("mpi3r_vals", 1, 0), ("mpi4r_vals", 1, 1),
+                                           ("mpi6r_vals", 1, 1), ("mpi8r_vals", 2, 1)])
+def test_contract_distributed_goldens(group, nd1, nd2):
+    for name, t1, d1, t2, d2, t3, d3, wires in contraction_cases(group):
+        r = ops.default_repls(len(d1), nd1, len(d2), nd2, wires)
+        _check_case(t1, d1, t2, d2, t3, d3, wires, r)
+
+
+# ------------------------------------------------------------------ contraction: GEMM route
+@pytest.mark.parametrize("da,db,wires", [
+    ([2] * 16, [2] * 16, [(i + 8, i) for i in range(8)]),                          # 256x256x256, no permutes
+    ([2] * 16, [2] * 16, [(0, 15), (3, 2), (9, 7), (12, 0), (5, 9), (14, 11), (7, 4), (1, 13)]),
+    ([3, 5, 7, 4, 9], [4, 9, 6, 11], [(3, 0), (4, 1)]),                             # odd sizes, tile edges
+    ([8, 16, 32], [32, 16, 40], [(2, 0), (1, 1)]),
+])
+def test_contract_gemm_route(da, db, wires):
+    a = rnd(int(np.prod(da)), 1)
+    b = rnd(int(np.prod(db)), 2)
+    out, dc, _ = ops.contract(dev(a), da, dev(b), db, wires)
+    ref, rdc = oracle.contract(a, da, b, db, wires)
+    got = host(out)
+    assert dc == rdc
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-12, err
+
+
+def test_contract_gemm_route_placement():
+    """GEMM route with a non-natural placement: the output permute follows repl1/repl2."""
+    da, db = [2] * 14, [2] * 14
+    wires = [(i, 13 - i) for i in range(7)]
+    rng = random.Random(3)
+    perm = list(range(14))
+    rng.shuffle(perm)
+    # wires use t1 axes 0..6 and t2 axes 13..7; open t1 axes 7..13, open t2 axes 0..6
+    r1 = [X] * 7 + perm[:7]
+    r2 = perm[7:] + [X] * 7
+    a = rnd(1 << 14, 5)
+    b = rnd(1 << 14, 6)
+    out, dc, _ = ops.contract(dev(a), da, dev(b), db, wires, r1, r2)
+    ref, _ = oracle.contract(a, da, b, db, wires, r1, r2)
+    err = np.linalg.norm(host(out) - ref) / np.linalg.norm(ref)
+    assert err < 1e-12, err
+
+
+def test_kernels_are_ordered_on_the_torch_stream():
+    """Results are visible to torch work queued after them on the same stream,
+    with no explicit synchronisation (the context runs on torch's stream)."""
+    dims = [2] * 20
+    p = list(range(20))[::-1]
+    for _ in range(3):
+        x = ops.fill_(ops.empty(1 << 20), 9)
+        y = ops.permute(x, dims, p)
+        z = ops.permute(y, dims, p)
+        assert torch.equal(torch.view_as_real(z), torch.view_as_real(x))  # no synchronize() before this
+        ops.gate_apply_(x, dims, [19], [0, 1, 1, 0])  # X gate on the last axis
+        w = x.view(-1, 2).flip(1).reshape(-1)
+        assert torch.equal(torch.view_as_real(w), torch.view_as_real(z))
+
+
+def test_norm2_and_scale():
+    x = rnd(12345, 3)
+    t = dev(x)
+    assert abs(ops.norm2(t) - np.vdot(x, x).real) < 1e-9
+    ops.scale_(t, 0.5 - 2j)
+    y = host(t)
+    exp = (x.real * 0.5 - x.imag * -2.0) + 1j * (x.real * -2.0 + x.imag * 0.5)
+    assert same_bits(y, exp)
