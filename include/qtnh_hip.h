@@ -129,6 +129,14 @@ qtnh_status qtnh_norm2_c128(qtnh_ctx_t ctx, const void *x, uint64_t n, double *o
 /* x_i *= (re + i im) */
 qtnh_status qtnh_scale_c128(qtnh_ctx_t ctx, void *x, uint64_t n, double re, double im);
 
+/* ---- tuning (no reference counterpart) ---- */
+/* Process-wide tile parameters of the power-of-two permute kernel: starting
+ * source-low / result-low tile bits a, b (1..12), target tile bits (<= 12),
+ * tile walk order (0: source order, 1: page-local on both sides), non-temporal
+ * stores (0/1), persistent blocks per compute unit.  Results do not depend on
+ * them. */
+void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks_per_cu);
+
 #ifdef __cplusplus
 }
 #endif

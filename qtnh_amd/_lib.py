@@ -45,7 +45,7 @@ SIGNATURES = {
     "qtnh_contract_c128": [_p, _p, _i, _u64p, _p, _i, _u64p, _i, _u16p, _u16p, _u16p, _p, _i, _dp],
     "qtnh_norm2_c128": [_p, _p, _u64, _dp],
     "qtnh_scale_c128": [_p, _p, _u64, ctypes.c_double, ctypes.c_double],
-    "qtnh_tune_permute": [_i, _i],
+    "qtnh_tune_permute": [_i, _i, _i, _i, _i, _i],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None}

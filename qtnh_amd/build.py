@@ -97,12 +97,13 @@ def build_cpp_tests():
     if not srcs:
         return []
     os.makedirs(TEST_BIN, exist_ok=True)
+    _run([sys.executable, os.path.join(TEST_SRC, "gen_golden.py"), os.path.join(TEST_BIN, "golden_gen.hpp")])
     outs = []
 
     def one(src):
         exe = os.path.join(TEST_BIN, os.path.splitext(os.path.basename(src))[0])
-        _run(["g++"] + CXX_FLAGS + ["-I", TEST_SRC, src, "-o", exe, "-L" + LIB_DIR, "-lqtnh", "-lqtnh_hip",
-                                    "-Wl,-rpath," + LIB_DIR])
+        _run(["g++"] + CXX_FLAGS + ["-I", TEST_SRC, "-I", TEST_BIN, src, "-o", exe, "-L" + LIB_DIR, "-lqtnh", "-lqtnh_hip",
+                                    "-Wl,-rpath,$ORIGIN/../../../qtnh_amd/lib"])
         return exe
 
     with concurrent.futures.ThreadPoolExecutor(_jobs()) as ex:

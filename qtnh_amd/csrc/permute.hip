@@ -36,7 +36,19 @@ struct PermBitsArgs {
 
 __device__ inline uint32_t lds_slot(uint32_t t, int a) { return t ^ ((t >> a) & ((1u << a) - 1u)); }
 
-template <typename Off>
+typedef double double2_t __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ inline void store_c128(c128 *p, c128 v) {
+  if (NT) {
+    double2_t w = {v.re, v.im};
+    __builtin_nontemporal_store(w, reinterpret_cast<double2_t *>(p));
+  } else {
+    *p = v;
+  }
+}
+
+template <typename Off, bool NT>
 __global__ void __launch_bounds__(kPermThreads) permute_bits_kernel(const c128 *__restrict__ src,
                                                                     c128 *__restrict__ dst, PermBitsArgs args,
                                                                     uint64_t n_tiles) {
@@ -83,7 +95,7 @@ __global__ void __launch_bounds__(kPermThreads) permute_bits_kernel(const c128 *
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kMaxEpt; ++r)
-      if (r < ept && threadIdx.x + r * kPermThreads < T) d[yo[r]] = tile[lt[r]];
+      if (r < ept && threadIdx.x + r * kPermThreads < T) store_c128<NT>(d + yo[r], tile[lt[r]]);
     __syncthreads();
   }
 }
@@ -147,12 +159,23 @@ __global__ void swap_general_kernel(c128 *__restrict__ buf, uint64_t n, uint64_t
 
 }  // namespace
 
-// Tunables for the bit-permute tile (source-low bits, result-low bits).
-static int g_perm_a = 6, g_perm_b = 6;
+// Tunables of the bit-permute tile.  a / b: starting source-low / result-low
+// bits; the tile then grows until it holds u_target bits (<= 12, 64 KiB of
+// LDS).  order 0 walks tiles by source bit position, 1 by the larger of the
+// source and result positions (page locality on both sides).  nt: stores
+// bypass the caches.  blocks_per_cu: persistent blocks per compute unit.
+struct PermTune {
+  int a = 6, b = 6, u_target = 12, order = 0, nt = 0, blocks_per_cu = 16;
+};
+static PermTune g_perm;
 
-extern "C" void qtnh_tune_permute(int a, int b) {
-  g_perm_a = std::max(1, std::min(a, 6));
-  g_perm_b = std::max(1, std::min(b, 6));
+extern "C" void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks_per_cu) {
+  g_perm.a = std::max(1, std::min(a, 12));
+  g_perm.b = std::max(1, std::min(b, 12));
+  g_perm.u_target = std::max(1, std::min(u_target, 12));
+  g_perm.order = order;
+  g_perm.nt = nt;
+  g_perm.blocks_per_cu = std::max(1, blocks_per_cu);
 }
 
 static qtnh_status launch_copy(qtnh_ctx_t ctx, const c128 *src, c128 *dst, uint64_t n) {
@@ -160,6 +183,20 @@ static qtnh_status launch_copy(qtnh_ctx_t ctx, const c128 *src, c128 *dst, uint6
   hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src, dst, n);
   return check_hip(ctx, hipGetLastError(), "copy launch");
 }
+
+namespace {
+// Tile bits for source-low a and result-low b: source bits [0, a) plus the
+// source bits of result bits [0, b).
+int tile_bits(const int *inv, int a, int b) {
+  int u = a;
+  for (int p = 0; p < b; ++p) u += inv[p] >= a;
+  return u;
+}
+bool tile_ok(const int *inv, int a, int b, int nb) {
+  int u = tile_bits(inv, a, b);
+  return a <= nb && b <= nb && u <= 12 && u - a <= 6 && u - b <= 6;
+}
+}  // namespace
 
 qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
                            const uint16_t *ptup) {
@@ -183,7 +220,16 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
     for (int p = 0; p < nb; ++p) inv[sigma[p]] = p;
     PermBitsArgs args;
     memset(&args, 0, sizeof(args));
-    int a = std::min(g_perm_a, nb), b = std::min(g_perm_b, nb);
+    int a = std::min(g_perm.a, nb), b = std::min(g_perm.b, nb);
+    if (!tile_ok(inv, a, b, nb)) a = b = std::min(6, nb);
+    // Grow the tile while it is smaller than the target (low bits kept in place,
+    // or a low result bit taken from a high source bit, give small tiles).
+    while (tile_bits(inv, a, b) < g_perm.u_target) {
+      int na = std::min(a + 1, nb), nbb = std::min(b + 1, nb);
+      if ((na == a && nbb == b) || !tile_ok(inv, na, nbb, nb)) break;
+      a = na;
+      b = nbb;
+    }
     // Tile bits: source bits [0, a) then the source bits of result bits [0, b) not already in.
     int extra[6], ne = 0;
     for (int p = 0; p < b; ++p)
@@ -204,7 +250,10 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
     // remaining tile bits (source bits in [0, a) whose result bit >= b), by result position.
     int rest[6], nr = 0;
     for (int p = b; p < nb; ++p)
-      if (inv[p] < a) rest[nr++] = p;  // result positions
+      if (inv[p] < a) {
+        if (nr == 6) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
+        rest[nr++] = p;  // result positions
+      }
     if (b + nr != u) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
     for (int j = 0; j < (1 << nr); ++j) {
       uint64_t off = 0;
@@ -225,12 +274,16 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
       args.s2t_lo[j] = (uint16_t)lo;
       args.s2t_hi[j] = (uint16_t)hi;
     }
-    int no = 0;
-    for (int p = 0; p < nb; ++p) {
-      if (tpos[p] >= 0) continue;
-      args.ob_in[no] = (uint8_t)p;
-      args.ob_out[no] = (uint8_t)sigma[p];
-      ++no;
+    // Outer (tile index) bits, fastest first.
+    int outer[64], no = 0;
+    for (int p = 0; p < nb; ++p)
+      if (tpos[p] < 0) outer[no++] = p;
+    if (g_perm.order == 1)
+      std::stable_sort(outer, outer + no,
+                       [&](int x, int y) { return std::max(x, sigma[x]) < std::max(y, sigma[y]); });
+    for (int j = 0; j < no; ++j) {
+      args.ob_in[j] = (uint8_t)outer[j];
+      args.ob_out[j] = (uint8_t)sigma[outer[j]];
     }
     args.a = a;
     args.b = b;
@@ -239,14 +292,21 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
     if (u > 12 || u < 0 || no + u != nb) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
     const uint64_t n_tiles = 1ull << no;
     const size_t lds = (size_t)sizeof(c128) << u;
-    const unsigned cap = (unsigned)std::max(1, ctx->num_cus) * 16;
+    const unsigned cap = (unsigned)std::max(1, ctx->num_cus) * (unsigned)g_perm.blocks_per_cu;
     unsigned blocks = (unsigned)std::min<uint64_t>(n_tiles, cap);
-    if (nb <= 32)
-      hipLaunchKernelGGL(permute_bits_kernel<uint32_t>, dim3(blocks), dim3(kPermThreads), lds, ctx->stream, src, dst,
-                         args, n_tiles);
+    const bool narrow = nb <= 32;
+    if (narrow && !g_perm.nt)
+      hipLaunchKernelGGL((permute_bits_kernel<uint32_t, false>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
+                         src, dst, args, n_tiles);
+    else if (narrow)
+      hipLaunchKernelGGL((permute_bits_kernel<uint32_t, true>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
+                         src, dst, args, n_tiles);
+    else if (!g_perm.nt)
+      hipLaunchKernelGGL((permute_bits_kernel<uint64_t, false>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
+                         src, dst, args, n_tiles);
     else
-      hipLaunchKernelGGL(permute_bits_kernel<uint64_t>, dim3(blocks), dim3(kPermThreads), lds, ctx->stream, src, dst,
-                         args, n_tiles);
+      hipLaunchKernelGGL((permute_bits_kernel<uint64_t, true>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
+                         src, dst, args, n_tiles);
     return check_hip(ctx, hipGetLastError(), "permute_bits launch");
   }
 

@@ -82,7 +82,32 @@ def _perm_cases():
         cases.append(([2] * rank, list(range(rank))))
         cases.append(([2] * rank, list(range(rank - 2)) + [rank - 1, rank - 2]))
         cases.append(([2] * rank, [1, 0] + list(range(2, rank))))
+        # low source bits kept in place (the tile grows past the kept bits)
+        hi = list(range(rank - 8))
+        pyr.shuffle(hi)
+        cases.append(([2] * rank, hi + list(range(rank - 8, rank))))
+        # lowest bit exchanged with the highest
+        sw = list(range(rank))
+        sw[0], sw[rank - 1] = rank - 1, 0
+        cases.append(([2] * rank, sw))
     return cases
+
+
+def test_permute_every_tile_shape():
+    """Results do not depend on the tile parameters."""
+    dims = [2] * 16
+    rng = random.Random(16)
+    p = list(range(16))
+    rng.shuffle(p)
+    x = rnd(1 << 16, 16)
+    ref = oracle.permute(x, dims, p)
+    try:
+        for a, b, u, order, nt, bpc in [(6, 6, 12, 0, 0, 16), (4, 4, 8, 1, 1, 2), (6, 3, 9, 1, 0, 4),
+                                        (3, 6, 12, 0, 1, 8), (2, 2, 4, 0, 0, 1), (8, 8, 12, 1, 1, 16)]:
+            ops.tune_permute(a, b, u, order, nt, bpc)
+            assert same_bits(host(ops.permute(dev(x), dims, p)), ref), (a, b, u, order, nt, bpc)
+    finally:
+        ops.tune_permute()
 
 
 @pytest.mark.parametrize("case", _perm_cases(), ids=lambda c: "%s-%s" % (c[0][:3], c[1][:3]))
