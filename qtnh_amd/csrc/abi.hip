@@ -37,7 +37,7 @@ qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out) {
     // The old workspace may still be read by queued kernels.
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return check_hip(ctx, e, "workspace sync");
-    hipFree(ctx->ws);
+    (void)hipFree(ctx->ws);
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
   }
@@ -284,12 +284,12 @@ qtnh_status qtnh_ctx_create(int device, void *stream, qtnh_ctx_t *out) {
 
 qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx) {
   if (!ctx) return QTNH_EINVAL;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  if (ctx->ws) hipFree(ctx->ws);
-  if (ctx->dev_scalar) hipFree(ctx->dev_scalar);
-  if (ctx->host_scalar) hipHostFree(ctx->host_scalar);
-  if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+  // Teardown is best effort: errors here cannot be reported to anyone useful.
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->dev_scalar) (void)hipFree(ctx->dev_scalar);
+  if (ctx->host_scalar) (void)hipHostFree(ctx->host_scalar);
   delete ctx;
   return QTNH_OK;
 }

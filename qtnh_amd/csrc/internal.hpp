@@ -12,8 +12,7 @@
 
 struct qtnh_ctx_s {
   int device = 0;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
+  hipStream_t stream = nullptr;  // caller's stream (not owned)
   int num_cus = 0;
   std::string last_error;
   // Growable device workspace for permute-to-matrix temporaries.

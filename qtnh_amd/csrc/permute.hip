@@ -21,7 +21,6 @@ namespace qtnh_hip {
 namespace {
 
 constexpr int kPermThreads = 256;
-constexpr int kMaxEpt = 16;  // tile elements per thread at 2^12 elements / 256 threads
 
 struct PermBitsArgs {
   uint64_t in_hi[64];   // source offset of the tile bits above a, indexed by t >> a
@@ -39,6 +38,14 @@ __device__ inline uint32_t lds_slot(uint32_t t, int a) { return t ^ ((t >> a) & 
 typedef double double2_t __attribute__((ext_vector_type(2)));
 
 template <bool NT>
+__device__ inline void store_d2(c128 *p, double2_t v) {
+  if (NT)
+    __builtin_nontemporal_store(v, reinterpret_cast<double2_t *>(p));
+  else
+    *reinterpret_cast<double2_t *>(p) = v;
+}
+
+template <bool NT>
 __device__ inline void store_c128(c128 *p, c128 v) {
   if (NT) {
     double2_t w = {v.re, v.im};
@@ -48,55 +55,92 @@ __device__ inline void store_c128(c128 *p, c128 v) {
   }
 }
 
-template <typename Off, bool NT>
-__global__ void __launch_bounds__(kPermThreads) permute_bits_kernel(const c128 *__restrict__ src,
-                                                                    c128 *__restrict__ dst, PermBitsArgs args,
-                                                                    uint64_t n_tiles) {
+// Tile and outer offsets of tile g.
+__device__ inline void tile_bases(const PermBitsArgs &args, uint64_t g, uint64_t &xb, uint64_t &yb) {
+  xb = 0;
+  yb = 0;
+  for (int j = 0; j < args.n_outer; ++j) {
+    const uint64_t bit = (g >> j) & 1ull;
+    xb |= bit << args.ob_in[j];
+    yb |= bit << args.ob_out[j];
+  }
+}
+
+// Persistent blocks walk tiles of 2^U elements, EPT elements per thread with
+// no per-element branches.  The loads of the next tile are issued before the
+// current tile is stored, so reads and writes overlap inside each block.  A
+// 2^12-element tile takes 64 KiB of LDS: at most two blocks (two waves per
+// SIMD) are resident, which the register budget below assumes.
+template <int U, typename Off, bool NT>
+__global__ void __launch_bounds__(kPermThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
+permute_bits_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, PermBitsArgs args, uint64_t n_tiles) {
+  constexpr uint32_t T = 1u << U;
+  constexpr int EPT = T >= kPermThreads ? (int)(T / kPermThreads) : 1;
   extern __shared__ c128 tile[];
-  const uint32_t T = 1u << args.u;
+  // Small tiles (T < threads): spare threads skip the data movement but keep
+  // taking part in the barriers.
+  const bool live = T >= kPermThreads || threadIdx.x < T;
   const uint32_t amask = (1u << args.a) - 1u;
   const uint32_t bmask = (1u << args.b) - 1u;
-  const int ept = (int)((T + kPermThreads - 1) / kPermThreads);
   // Per-thread tile offsets are the same for every tile: compute them once.
-  Off xo[kMaxEpt], yo[kMaxEpt];
-  uint16_t ls[kMaxEpt], lt[kMaxEpt];
+  // Narrow tensors (< 2^28 elements) keep 32-bit byte offsets against a
+  // uniform base pointer; LDS slots of the load and store phases share a register.
+  Off xo[EPT], yo[EPT];
+  uint32_t slots[EPT];
+  constexpr Off kScale = sizeof(Off) == 4 ? (Off)sizeof(c128) : (Off)1;
 #pragma unroll
-  for (int r = 0; r < kMaxEpt; ++r) {
-    uint32_t e = threadIdx.x + r * kPermThreads;
-    if (r < ept && e < T) {
-      xo[r] = (Off)((e & amask) | args.in_hi[e >> args.a]);
-      ls[r] = (uint16_t)lds_slot(e, args.a);
-      uint32_t t = args.s2t_lo[e & 63u] | args.s2t_hi[e >> 6];
-      yo[r] = (Off)((e & bmask) | args.out_hi[e >> args.b]);
-      lt[r] = (uint16_t)lds_slot(t, args.a);
-    } else {
-      xo[r] = 0;
-      yo[r] = 0;
-      ls[r] = 0;
-      lt[r] = 0;
-    }
+  for (int r = 0; r < EPT; ++r) {
+    const uint32_t e = live ? threadIdx.x + r * kPermThreads : 0;
+    xo[r] = (Off)((e & amask) | args.in_hi[e >> args.a]) * kScale;
+    const uint32_t t = args.s2t_lo[e & 63u] | args.s2t_hi[e >> 6];
+    yo[r] = (Off)((e & bmask) | args.out_hi[e >> args.b]) * kScale;
+    slots[r] = lds_slot(e, args.a) | (lds_slot(t, args.a) << 16);
   }
-  for (uint64_t g = blockIdx.x; g < n_tiles; g += gridDim.x) {
-    uint64_t xb = 0, yb = 0;
-    for (int j = 0; j < args.n_outer; ++j) {
-      uint64_t bit = (g >> j) & 1ull;
-      xb |= bit << args.ob_in[j];
-      yb |= bit << args.ob_out[j];
-    }
+  auto at = [](const c128 *base, Off off) -> const double2_t * {
+    if (sizeof(Off) == 4) return reinterpret_cast<const double2_t *>(reinterpret_cast<const char *>(base) + off);
+    return reinterpret_cast<const double2_t *>(base + off);
+  };
+  auto at_mut = [](c128 *base, Off off) -> c128 * {
+    if (sizeof(Off) == 4) return reinterpret_cast<c128 *>(reinterpret_cast<char *>(base) + off);
+    return base + off;
+  };
+  double2_t *ltile = reinterpret_cast<double2_t *>(tile);
+  uint64_t g = blockIdx.x;
+  if (g >= n_tiles) return;  // uniform: the whole block leaves
+  uint64_t xb, yb;
+  tile_bases(args, g, xb, yb);
+  double2_t v[EPT];
+  if (live) {
     const c128 *s = src + xb;
-    c128 *d = dst + yb;
-    c128 v[kMaxEpt];
 #pragma unroll
-    for (int r = 0; r < kMaxEpt; ++r)
-      if (r < ept && threadIdx.x + r * kPermThreads < T) v[r] = s[xo[r]];
+    for (int r = 0; r < EPT; ++r) v[r] = *at(s, xo[r]);
+  }
+  while (true) {
+    if (live) {
 #pragma unroll
-    for (int r = 0; r < kMaxEpt; ++r)
-      if (r < ept && threadIdx.x + r * kPermThreads < T) tile[ls[r]] = v[r];
+      for (int r = 0; r < EPT; ++r) ltile[slots[r] & 0xffffu] = v[r];
+    }
     __syncthreads();
+    const uint64_t gn = g + gridDim.x;
+    const bool more = gn < n_tiles;  // uniform
+    uint64_t xn = 0, yn = 0;
+    if (more) {
+      tile_bases(args, gn, xn, yn);
+      if (live) {
+        const c128 *s = src + xn;
 #pragma unroll
-    for (int r = 0; r < kMaxEpt; ++r)
-      if (r < ept && threadIdx.x + r * kPermThreads < T) store_c128<NT>(d + yo[r], tile[lt[r]]);
+        for (int r = 0; r < EPT; ++r) v[r] = *at(s, xo[r]);
+      }
+    }
+    if (live) {
+      c128 *d = dst + yb;
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) store_d2<NT>(at_mut(d, yo[r]), ltile[slots[r] >> 16]);
+    }
     __syncthreads();
+    if (!more) break;
+    g = gn;
+    yb = yn;
   }
 }
 
@@ -164,8 +208,10 @@ __global__ void swap_general_kernel(c128 *__restrict__ buf, uint64_t n, uint64_t
 // LDS).  order 0 walks tiles by source bit position, 1 by the larger of the
 // source and result positions (page locality on both sides).  nt: stores
 // bypass the caches.  blocks_per_cu: persistent blocks per compute unit.
+// Defaults measured on MI355X (rank-28 tensors): 5.1-5.2 TB/s over random,
+// reversed, low-bits-kept and bit-0-to-top permutations.
 struct PermTune {
-  int a = 6, b = 6, u_target = 12, order = 0, nt = 0, blocks_per_cu = 16;
+  int a = 6, b = 6, u_target = 12, order = 1, nt = 1, blocks_per_cu = 2;
 };
 static PermTune g_perm;
 
@@ -185,6 +231,32 @@ static qtnh_status launch_copy(qtnh_ctx_t ctx, const c128 *src, c128 *dst, uint6
 }
 
 namespace {
+struct PermLaunch {
+  hipStream_t stream;
+  unsigned blocks;
+  size_t lds;
+  const c128 *src;
+  c128 *dst;
+  uint64_t n_tiles;
+  bool narrow, nt;
+};
+
+template <int U>
+void launch_tile(const PermLaunch &L, const PermBitsArgs &args) {
+  if (L.narrow && !L.nt)
+    hipLaunchKernelGGL((permute_bits_kernel<U, uint32_t, false>), dim3(L.blocks), dim3(kPermThreads), L.lds, L.stream,
+                       L.src, L.dst, args, L.n_tiles);
+  else if (L.narrow)
+    hipLaunchKernelGGL((permute_bits_kernel<U, uint32_t, true>), dim3(L.blocks), dim3(kPermThreads), L.lds, L.stream,
+                       L.src, L.dst, args, L.n_tiles);
+  else if (!L.nt)
+    hipLaunchKernelGGL((permute_bits_kernel<U, uint64_t, false>), dim3(L.blocks), dim3(kPermThreads), L.lds, L.stream,
+                       L.src, L.dst, args, L.n_tiles);
+  else
+    hipLaunchKernelGGL((permute_bits_kernel<U, uint64_t, true>), dim3(L.blocks), dim3(kPermThreads), L.lds, L.stream,
+                       L.src, L.dst, args, L.n_tiles);
+}
+
 // Tile bits for source-low a and result-low b: source bits [0, a) plus the
 // source bits of result bits [0, b).
 int tile_bits(const int *inv, int a, int b) {
@@ -294,19 +366,22 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
     const size_t lds = (size_t)sizeof(c128) << u;
     const unsigned cap = (unsigned)std::max(1, ctx->num_cus) * (unsigned)g_perm.blocks_per_cu;
     unsigned blocks = (unsigned)std::min<uint64_t>(n_tiles, cap);
-    const bool narrow = nb <= 32;
-    if (narrow && !g_perm.nt)
-      hipLaunchKernelGGL((permute_bits_kernel<uint32_t, false>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
-                         src, dst, args, n_tiles);
-    else if (narrow)
-      hipLaunchKernelGGL((permute_bits_kernel<uint32_t, true>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
-                         src, dst, args, n_tiles);
-    else if (!g_perm.nt)
-      hipLaunchKernelGGL((permute_bits_kernel<uint64_t, false>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
-                         src, dst, args, n_tiles);
-    else
-      hipLaunchKernelGGL((permute_bits_kernel<uint64_t, true>), dim3(blocks), dim3(kPermThreads), lds, ctx->stream,
-                         src, dst, args, n_tiles);
+    PermLaunch L{ctx->stream, blocks, lds, src, dst, n_tiles, nb <= 28, g_perm.nt != 0};
+    switch (u) {
+      case 0: launch_tile<0>(L, args); break;
+      case 1: launch_tile<1>(L, args); break;
+      case 2: launch_tile<2>(L, args); break;
+      case 3: launch_tile<3>(L, args); break;
+      case 4: launch_tile<4>(L, args); break;
+      case 5: launch_tile<5>(L, args); break;
+      case 6: launch_tile<6>(L, args); break;
+      case 7: launch_tile<7>(L, args); break;
+      case 8: launch_tile<8>(L, args); break;
+      case 9: launch_tile<9>(L, args); break;
+      case 10: launch_tile<10>(L, args); break;
+      case 11: launch_tile<11>(L, args); break;
+      default: launch_tile<12>(L, args); break;
+    }
     return check_hip(ctx, hipGetLastError(), "permute_bits launch");
   }
 

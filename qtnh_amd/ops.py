@@ -213,6 +213,6 @@ def scale_(x, c):
     return x
 
 
-def tune_permute(a=6, b=6, u_target=12, order=0, nt=0, blocks_per_cu=16):
+def tune_permute(a=6, b=6, u_target=12, order=1, nt=1, blocks_per_cu=2):
     """Tile parameters of the power-of-two permute kernel (process-wide)."""
     lib().qtnh_tune_permute(a, b, u_target, order, nt, blocks_per_cu)
