@@ -93,6 +93,25 @@ def cpu_baseline(args, ptup_full):
     }
 
 
+_KERNEL_PREFIX = {"permute": "permute_bits_kernel", "gate": "gate_", "diag": "diag_bits_kernel"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a
+    plain 4 GiB copy); None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for name, v in data.get("kernels", {}).items():
+        if name.startswith(_KERNEL_PREFIX[kernel]):
+            return v["bytes"]
+    return None
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -193,7 +212,7 @@ def main():
         "hbm_pct_peak": round(100.0 * 3 * bytes_per / ms_step / 1e6 / HBM_PEAK_GBS, 2),
         "kernels": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in kernels.items()},
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dominant),
                      "algorithmic_bytes_per_launch": bytes_per},
         "wall_s_timed": round(wall, 3),
     }
