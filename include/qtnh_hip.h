@@ -115,9 +115,12 @@ qtnh_status qtnh_diag_apply_c128(qtnh_ctx_t ctx, void *state, int rank, const ui
 /* C = sum over wires of A*B, result placed by repl1/repl2 (QTNH_X for wired
  * axes).  Small problems use a kernel that keeps the reference summation
  * order; large ones permute to matrices and run a complex GEMM on MFMA.
- * precision QTNH_PREC_FP64 (the drop-in) or QTNH_PREC_SPLIT; err_out (may
- * be NULL) receives an a-posteriori relative error estimate for SPLIT and 0
- * for FP64.  A, B and C must not overlap. */
+ * precision QTNH_PREC_FP64 (the drop-in) or QTNH_PREC_SPLIT: GEMM-sized
+ * problems then run as exact INT8 matrix-core products of 7-bit slices of A
+ * and B recombined in FP64 (finite inputs only).  err_out (may be NULL)
+ * receives eta of the componentwise a-priori bound
+ *   |C_ij - C~_ij| <= eta * max_k|A_ik| * max_k|B_kj|   (max over re and im)
+ * for SPLIT, and 0 when the FP64 kernels ran.  A, B and C must not overlap. */
 qtnh_status qtnh_contract_c128(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B,
                                int rB, const uint64_t *dB, int nw, const uint16_t *wires,
                                const uint16_t *repl1, const uint16_t *repl2, void *C, int precision,
@@ -136,6 +139,11 @@ qtnh_status qtnh_scale_c128(qtnh_ctx_t ctx, void *x, uint64_t n, double re, doub
  * stores (0/1), persistent blocks per compute unit.  Results do not depend on
  * them. */
 void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks_per_cu);
+/* Block width (64 or 128 complex columns, 0: chosen by size) of the FP64
+ * complex GEMM. */
+void qtnh_tune_zgemm(int bn);
+/* Number of 7-bit slices (1..8, default 6) of QTNH_PREC_SPLIT contractions. */
+void qtnh_tune_split(int slices);
 
 #ifdef __cplusplus
 }

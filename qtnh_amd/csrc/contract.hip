@@ -250,18 +250,27 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
                             const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *r1, const uint16_t *r2,
                             c128 *C, int precision, double *err_out) {
   Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
-  if (precision == QTNH_PREC_SPLIT)
-    return set_error(ctx, QTNH_EUNSUPPORTED, "contract: split precision is not built yet");
-  if (use_direct(P)) return launch_contract_direct(ctx, A, rA, dA, B, rB, dB, nw, wires, r1, r2, C);
+  if (use_direct(P)) {
+    // Small problems run exactly in FP64 whatever the precision asked for.
+    if (err_out) *err_out = 0.0;
+    return launch_contract_direct(ctx, A, rA, dA, B, rB, dB, nw, wires, r1, r2, C);
+  }
+  const bool split = precision == QTNH_PREC_SPLIT;
 
-  // GEMM route.  A' = A permuted to [open1, closed]; B' = B permuted to [closed, open2].
+  // GEMM route.  A' = A permuted to [open1, closed]; B' = B permuted to
+  // [closed, open2] (FP64) or [open2, closed] (split, which reads B by columns).
   std::vector<uint16_t> pA(rA), pB(rB), pC(P.rC);
   int q = 0;
   for (int i : P.open1) pA[i] = (uint16_t)q++;
   for (int i : P.cA) pA[i] = (uint16_t)q++;
   q = 0;
-  for (int i : P.cB) pB[i] = (uint16_t)q++;
-  for (int i : P.open2) pB[i] = (uint16_t)q++;
+  if (split) {
+    for (int i : P.open2) pB[i] = (uint16_t)q++;
+    for (int i : P.cB) pB[i] = (uint16_t)q++;
+  } else {
+    for (int i : P.cB) pB[i] = (uint16_t)q++;
+    for (int i : P.open2) pB[i] = (uint16_t)q++;
+  }
   bool idA = true, idB = true, idC = true;
   for (int i = 0; i < rA; ++i) idA = idA && pA[i] == i;
   for (int i = 0; i < rB; ++i) idB = idB && pB[i] == i;
@@ -271,7 +280,8 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
   }
   const uint64_t nA = P.M * P.K, nB = P.K * P.N;
   auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
-  const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC));
+  const uint64_t sbytes = split ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull) : 0;
+  const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC)) + sbytes;
   char *ws = nullptr;
   if (bytes) {
     void *w;
@@ -300,7 +310,13 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     Cn = (c128 *)(ws + off);
     off += pad(P.nC);
   }
-  qtnh_status st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N);
+  qtnh_status st;
+  if (split) {
+    st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + off, sbytes, err_out);
+  } else {
+    st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N);
+    if (err_out) *err_out = 0.0;
+  }
   if (st) return st;
   if (!idC) {
     // Natural result dims: open1 dims then open2 dims.
@@ -310,7 +326,6 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     st = launch_permute(ctx, Cn, C, P.rC, dn.data(), pC.data());
     if (st) return st;
   }
-  if (err_out) *err_out = 0.0;
   return QTNH_OK;
 }
 

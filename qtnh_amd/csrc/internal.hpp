@@ -73,6 +73,13 @@ qtnh_status launch_scale(qtnh_ctx_t ctx, c128 *x, uint64_t n, double re, double 
 qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N,
                              uint64_t K, uint64_t lda, uint64_t ldb, uint64_t ldc);
 
+// Split-precision complex GEMM C[MxN] = A[MxK] * Bt[NxK]^T (all contiguous,
+// row-major) on INT8 matrix cores; scratch of split_scratch_bytes(M, N, K).
+qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
+                               uint64_t K, void *scratch, uint64_t scratch_bytes, double *err_out);
+uint64_t split_scratch_bytes(uint64_t M, uint64_t N, uint64_t K);
+int split_slices();
+
 // Host helpers for placement (also exported through the ABI).
 qtnh_status default_repls(int r1, int ndis1, int r2, int ndis2, int nw, const uint16_t *wires, uint16_t *repl1,
                           uint16_t *repl2);

@@ -4,8 +4,11 @@
 // Each 16x16 complex output tile keeps a real and an imaginary accumulator
 // and takes four v_mfma_f64_16x16x4_f64 per 4-deep k step:
 //   Cr += Ar*Br, Cr += (-Ai)*Bi, Ci += Ar*Bi, Ci += Ai*Br.
-// A workgroup of 4 waves owns a 64x64 complex tile (each wave 32x32 = 2x2
-// MFMA tiles); operands are staged through LDS 16 complex deep.
+// A workgroup of 4 waves owns a BM x BN complex tile, each wave a 32 x (BN/2)
+// sub-tile (2 x BN/32 MFMA tiles).  Operands are staged through LDS BK complex
+// deep; the global loads of stage s+1 are issued into registers before the
+// MFMAs of stage s, so memory latency hides behind the (slow, 64-cycle) f64
+// MFMAs.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -14,66 +17,99 @@ namespace qtnh_hip {
 namespace {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef double double2_t __attribute__((ext_vector_type(2)));
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BK = 16;
 constexpr int kThreads = 256;
-constexpr int LDA_S = BK + 1;  // padded LDS row of the A tile (complex elements)
-constexpr int LDB_S = BN + 1;  // padded LDS row of the B tile
 
+template <int BM, int BN>
+struct ZTile {
+  static constexpr int LDA = BK + 1;  // padded LDS rows (complex elements)
+  static constexpr int LDB = BN + 1;
+  static constexpr int A_PER = BM * BK / kThreads;  // complex loads per thread per stage
+  static constexpr int B_PER = BK * BN / kThreads;
+  static constexpr int WN = BN / 2;                 // wave tile: 32 x WN
+  static constexpr int TN = WN / 16;                // MFMA tiles along n per wave
+  static_assert(BM == 64, "two waves of 32 rows");
+  static_assert(A_PER >= 1 && B_PER >= 1, "tile too small");
+};
+
+template <int BM, int BN>
 __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restrict__ A, const c128 *__restrict__ B,
                                                              c128 *__restrict__ C, uint64_t M, uint64_t N, uint64_t K,
                                                              uint64_t lda, uint64_t ldb, uint64_t ldc) {
-  __shared__ c128 As[BM * LDA_S];
-  __shared__ c128 Bs[BK * LDB_S];
+  using Z = ZTile<BM, BN>;
+  __shared__ double2_t As[BM * Z::LDA];
+  __shared__ double2_t Bs[BK * Z::LDB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32;  // wave tile origin inside the block tile
-  const int wn = (wave & 1) * 32;
+  const int wn = (wave & 1) * Z::WN;
   const uint64_t ntn = (N + BN - 1) / BN;
   const uint64_t m0 = ((uint64_t)blockIdx.x / ntn) * BM;
   const uint64_t n0 = ((uint64_t)blockIdx.x % ntn) * BN;
 
-  double4_t accR[2][2], accI[2][2];
+  double4_t accR[2][Z::TN], accI[2][Z::TN];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < Z::TN; ++j) {
       accR[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
       accI[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
     }
 
-  const c128 zero = {0.0, 0.0};
-  for (uint64_t k0 = 0; k0 < K; k0 += BK) {
-    // Stage A (64 x 16) and B (16 x 64): 4 complex elements per thread each.
+  const double2_t zero = {0.0, 0.0};
+  const double2_t *A2 = reinterpret_cast<const double2_t *>(A);
+  const double2_t *B2 = reinterpret_cast<const double2_t *>(B);
+  double2_t pa[Z::A_PER], pb[Z::B_PER];
+  auto fetch = [&](uint64_t k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int idx = tid + r * kThreads;
-      int ar = idx / BK, ac = idx % BK;
-      uint64_t gm = m0 + ar, gk = k0 + ac;
-      As[ar * LDA_S + ac] = (gm < M && gk < K) ? A[gm * lda + gk] : zero;
-      int br = idx / BN, bc = idx % BN;
-      uint64_t gk2 = k0 + br, gn = n0 + bc;
-      Bs[br * LDB_S + bc] = (gk2 < K && gn < N) ? B[gk2 * ldb + gn] : zero;
+    for (int r = 0; r < Z::A_PER; ++r) {
+      const int idx = tid + r * kThreads;
+      const int ar = idx / BK, ac = idx % BK;
+      const uint64_t gm = m0 + ar, gk = k0 + ac;
+      pa[r] = (gm < M && gk < K) ? A2[gm * lda + gk] : zero;
+    }
+#pragma unroll
+    for (int r = 0; r < Z::B_PER; ++r) {
+      const int idx = tid + r * kThreads;
+      const int br = idx / BN, bc = idx % BN;
+      const uint64_t gk = k0 + br, gn = n0 + bc;
+      pb[r] = (gk < K && gn < N) ? B2[gk * ldb + gn] : zero;
+    }
+  };
+  fetch(0);
+  for (uint64_t k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int r = 0; r < Z::A_PER; ++r) {
+      const int idx = tid + r * kThreads;
+      As[(idx / BK) * Z::LDA + idx % BK] = pa[r];
+    }
+#pragma unroll
+    for (int r = 0; r < Z::B_PER; ++r) {
+      const int idx = tid + r * kThreads;
+      Bs[(idx / BN) * Z::LDB + idx % BN] = pb[r];
     }
     __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);  // in flight during the MFMAs below
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kl = kk + (lane >> 4);
-      c128 a[2], b[2];
+      double2_t a[2], b[Z::TN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[(wm + i * 16 + (lane & 15)) * LDA_S + kl];
+      for (int i = 0; i < 2; ++i) a[i] = As[(wm + i * 16 + (lane & 15)) * Z::LDA + kl];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kl * LDB_S + wn + j * 16 + (lane & 15)];
+      for (int j = 0; j < Z::TN; ++j) b[j] = Bs[kl * Z::LDB + wn + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const double nai = -a[i].im;
+        const double nai = -a[i].y;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].re, b[j].re, accR[i][j], 0, 0, 0);
-          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].im, accR[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].re, b[j].im, accI[i][j], 0, 0, 0);
-          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].im, b[j].re, accI[i][j], 0, 0, 0);
+        for (int j = 0; j < Z::TN; ++j) {
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, accR[i][j], 0, 0, 0);
+          accR[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, b[j].y, accR[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, accI[i][j], 0, 0, 0);
+          accI[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, accI[i][j], 0, 0, 0);
         }
       }
     }
@@ -84,11 +120,11 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < Z::TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        uint64_t gm = m0 + wm + i * 16 + (lane >> 4) + 4 * v;
-        uint64_t gn = n0 + wn + j * 16 + (lane & 15);
+        const uint64_t gm = m0 + wm + i * 16 + (lane >> 4) + 4 * v;
+        const uint64_t gn = n0 + wn + j * 16 + (lane & 15);
         if (gm < M && gn < N) {
           c128 out;
           out.re = accR[i][j][v];
@@ -100,12 +136,26 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
 
 }  // namespace
 
+// Block width: 0 picks by size (64 x 128 tiles run at occupancy 1 and win
+// while 64 x 64 tiles would number under 2048, about 8 per compute unit;
+// past that the occupancy-3 64 x 64 kernel hides latency better), or 64 / 128.
+static int g_zgemm_bn = 0;
+
+extern "C" void qtnh_tune_zgemm(int bn) { g_zgemm_bn = (bn == 64 || bn == 128) ? bn : 0; }
+
 qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N, uint64_t K,
                              uint64_t lda, uint64_t ldb, uint64_t ldc) {
-  const uint64_t tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  int bn = g_zgemm_bn;
+  if (bn == 0) bn = (((M + 63) / 64) * ((N + 63) / 64) < 2048) ? 128 : 64;
+  if (N < 128) bn = 64;
+  const uint64_t tiles = ((N + bn - 1) / bn) * ((M + 63) / 64);
   if (tiles > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "zgemm: grid too large");
-  hipLaunchKernelGGL(zgemm_f64_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, ctx->stream, A, B, C, M, N, K, lda,
-                     ldb, ldc);
+  if (bn == 128)
+    hipLaunchKernelGGL((zgemm_f64_kernel<64, 128>), dim3((unsigned)tiles), dim3(kThreads), 0, ctx->stream, A, B, C, M,
+                       N, K, lda, ldb, ldc);
+  else
+    hipLaunchKernelGGL((zgemm_f64_kernel<64, 64>), dim3((unsigned)tiles), dim3(kThreads), 0, ctx->stream, A, B, C, M, N,
+                       K, lda, ldb, ldc);
   return check_hip(ctx, hipGetLastError(), "zgemm launch");
 }
 

@@ -216,3 +216,13 @@ def scale_(x, c):
 def tune_permute(a=6, b=6, u_target=12, order=1, nt=1, blocks_per_cu=2):
     """Tile parameters of the power-of-two permute kernel (process-wide)."""
     lib().qtnh_tune_permute(a, b, u_target, order, nt, blocks_per_cu)
+
+
+def tune_zgemm(bn=0):
+    """Block width (64 or 128 complex columns; 0 chooses by size) of the FP64 complex GEMM (process-wide)."""
+    lib().qtnh_tune_zgemm(bn)
+
+
+def tune_split(slices=6):
+    """Number of 7-bit slices of split-precision contractions (process-wide)."""
+    lib().qtnh_tune_split(slices)
