@@ -132,6 +132,47 @@ qtnh_status qtnh_norm2_c128(qtnh_ctx_t ctx, const void *x, uint64_t n, double *o
 /* x_i *= (re + i im) */
 qtnh_status qtnh_scale_c128(qtnh_ctx_t ctx, void *x, uint64_t n, double re, double im);
 
+/* ---- circuits: gate sequences on a resident state (no single reference
+ * function: the reference applies a circuit as one Tensor::contract per gate,
+ * examples/qft.cpp:35-111, tests/gen/qft.hpp:10-111, through the symmetric
+ * in-place placement of contract.cpp:250-289).  Qubit q is axis q of a rank-n
+ * state with every dim 2 (axis 0 most significant). ---- */
+typedef struct qtnh_circuit_s *qtnh_circuit_t;
+#define QTNH_RUN_PER_GATE 0 /* one streaming launch per op (the reference's one contraction per gate) */
+#define QTNH_RUN_FUSED 1    /* ops grouped into runs of <= tile-bits qubits, one state pass per run */
+
+qtnh_status qtnh_circuit_create(int n_qubits, qtnh_circuit_t *out);
+qtnh_status qtnh_circuit_destroy(qtnh_circuit_t c);
+/* Appends a dense k-qubit gate (k <= 3), G[in][out] row-major, (2^k)^2
+ * complex, contracted over `in` like qtnh_gate_apply_c128. */
+qtnh_status qtnh_circuit_gate(qtnh_circuit_t c, int k, const uint16_t *qubits, const double *G);
+/* Appends a diagonal k-qubit gate (k <= 3), d of 2^k complex, like qtnh_diag_apply_c128. */
+qtnh_status qtnh_circuit_diag(qtnh_circuit_t c, int k, const uint16_t *qubits, const double *d);
+/* Appends the QFT of the reference's example and tests (tests/gen/qft.hpp:47-108):
+ * for i: H on i, then for j > i the controlled phase diag(1,1,1,e^{i pi/2^(j-i)})
+ * on (i, j).  phases[2m], phases[2m+1] hold e^{i pi/2^m} (m = 1..n-1); NULL
+ * computes them with cos/sin. */
+qtnh_status qtnh_circuit_qft(qtnh_circuit_t c, const double *phases);
+/* Appends `depth` layers of a seeded random circuit: a Haar-random SU(2) on
+ * every qubit, then CZ on the pairs of a seeded random pairing of the qubits. */
+qtnh_status qtnh_circuit_random(qtnh_circuit_t c, int depth, uint64_t seed);
+/* Number of qubits, ops, and runs under the current tile size (any may be NULL). */
+qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, uint64_t *n_runs);
+/* Reads op i back: kind (0 gate, 1 diagonal), k, qubits[k], coefficients. */
+qtnh_status qtnh_circuit_op(qtnh_circuit_t c, uint64_t i, int *kind, int *k, uint16_t *qubits, double *coef);
+/* Applies the circuit in place to a 2^n-amplitude state; both modes give
+ * bit-identical results. */
+qtnh_status qtnh_circuit_run(qtnh_ctx_t ctx, qtnh_circuit_t c, void *state, int mode);
+/* Applies circuit i to states[i] (distinct buffers) for every i, fused, with
+ * one launch per run index across the whole batch. */
+qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit_t *circuits, void *const *states);
+
+/* ---- queries on a state (no reference counterpart) ---- */
+/* out[j] = <psi| Z_{qubits[j]} |psi> = sum_x |psi_x|^2 (1 - 2 x_q), fixed
+ * reduction order; blocks until the values are on the host. */
+qtnh_status qtnh_expect_z_c128(qtnh_ctx_t ctx, const void *state, int n_qubits, int nq, const uint16_t *qubits,
+                               double *out);
+
 /* ---- tuning (no reference counterpart) ---- */
 /* Process-wide tile parameters of the power-of-two permute kernel: starting
  * source-low / result-low tile bits a, b (1..12), target tile bits (<= 12),
@@ -144,6 +185,8 @@ void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks
 void qtnh_tune_zgemm(int bn);
 /* Number of 7-bit slices (1..8, default 6) of QTNH_PREC_SPLIT contractions. */
 void qtnh_tune_split(int slices);
+/* Tile bits (3..13, default 12) of fused circuit runs. */
+void qtnh_tune_circuit(int tile_bits);
 
 #ifdef __cplusplus
 }

@@ -172,8 +172,11 @@ def contract(a, da, b, db, wires, repl1=None, repl2=None):
     return out, dc
 
 
-def gate_apply(x, dims, axes, g):
-    x = np.array(x, dtype=np.complex128, copy=True)
+def gate_apply(x, dims, axes, g, inplace=False):
+    """In-place k-axis gate (x'[..o..] = sum_i x[..i..] g[i, o]); a copy unless inplace
+    (then x must be a contiguous complex128 array)."""
+    if not inplace:
+        x = np.array(x, dtype=np.complex128, copy=True)
     d, dp = _u64(dims)
     a, ap = _u16(axes)
     g, gp = _c128(g)
@@ -181,8 +184,9 @@ def gate_apply(x, dims, axes, g):
     return x
 
 
-def diag_apply(x, dims, axes, dvals):
-    x = np.array(x, dtype=np.complex128, copy=True)
+def diag_apply(x, dims, axes, dvals, inplace=False):
+    if not inplace:
+        x = np.array(x, dtype=np.complex128, copy=True)
     d, dp = _u64(dims)
     a, ap = _u16(axes)
     dv, dvp = _c128(dvals)

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libqtnh_hip.so")
 QTNH_OK, QTNH_EINVAL, QTNH_ENOMEM, QTNH_EHIP, QTNH_EUNSUPPORTED = range(5)
 X = 0xFFFF
 PREC_FP64, PREC_SPLIT = 0, 1
+RUN_PER_GATE, RUN_FUSED = 0, 1
 DIST_NORMAL, DIST_UNIFORM = 0, 1
 
 _p = ctypes.c_void_p
@@ -48,10 +49,22 @@ SIGNATURES = {
     "qtnh_tune_permute": [_i, _i, _i, _i, _i, _i],
     "qtnh_tune_zgemm": [_i],
     "qtnh_tune_split": [_i],
+    "qtnh_tune_circuit": [_i],
+    "qtnh_circuit_create": [_i, ctypes.POINTER(_p)],
+    "qtnh_circuit_destroy": [_p],
+    "qtnh_circuit_gate": [_p, _i, _u16p, _dp],
+    "qtnh_circuit_diag": [_p, _i, _u16p, _dp],
+    "qtnh_circuit_qft": [_p, _dp],
+    "qtnh_circuit_random": [_p, _i, _u64],
+    "qtnh_circuit_info": [_p, ctypes.POINTER(_i), _u64p, _u64p],
+    "qtnh_circuit_op": [_p, _u64, ctypes.POINTER(_i), ctypes.POINTER(_i), _u16p, _dp],
+    "qtnh_circuit_run": [_p, _p, _p, _i],
+    "qtnh_circuit_run_batch": [_p, _i, ctypes.POINTER(_p), ctypes.POINTER(_p)],
+    "qtnh_expect_z_c128": [_p, _p, _i, _i, _u16p, _dp],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_zgemm": None,
-            "qtnh_tune_split": None}
+            "qtnh_tune_split": None, "qtnh_tune_circuit": None}
 
 _LIB = None
 

@@ -28,28 +28,35 @@ qtnh_status check_hip(qtnh_ctx_t ctx, hipError_t e, const char *what) {
                    hipGetErrorString(e));
 }
 
-qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out) {
-  if (bytes <= ctx->ws_bytes) {
-    *out = ctx->ws;
-    return QTNH_OK;
-  }
-  if (ctx->ws) {
-    // The old workspace may still be read by queued kernels.
+qtnh_status grow_buffer(qtnh_ctx_t ctx, void **buf, uint64_t *cap, uint64_t bytes) {
+  if (bytes <= *cap) return QTNH_OK;
+  if (*buf) {
+    // The old buffer may still be read by queued kernels.
     hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return check_hip(ctx, e, "workspace sync");
-    (void)hipFree(ctx->ws);
-    ctx->ws = nullptr;
-    ctx->ws_bytes = 0;
+    if (e != hipSuccess) return check_hip(ctx, e, "buffer sync");
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
   }
   uint64_t want = std::max<uint64_t>(bytes, 1ull << 20);
-  hipError_t e = hipMalloc(&ctx->ws, want);
+  hipError_t e = hipMalloc(buf, want);
   if (e != hipSuccess) {
-    ctx->ws = nullptr;
-    return check_hip(ctx, e, "workspace alloc");
+    *buf = nullptr;
+    return check_hip(ctx, e, "device buffer alloc");
   }
-  ctx->ws_bytes = want;
-  *out = ctx->ws;
+  *cap = want;
   return QTNH_OK;
+}
+
+qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out) {
+  qtnh_status st = grow_buffer(ctx, &ctx->ws, &ctx->ws_bytes, bytes);
+  *out = st ? nullptr : ctx->ws;
+  return st;
+}
+
+qtnh_status enter_ctx(qtnh_ctx_t ctx) {
+  if (!ctx) return QTNH_EINVAL;
+  return check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
 }
 
 bool checked_size(int rank, const uint64_t *dims, uint64_t *n) {
@@ -226,11 +233,7 @@ bool axes_valid(int rank, int k, const uint16_t *axes) {
 
 qtnh_status need_ctx(qtnh_ctx_t ctx) { return ctx ? QTNH_OK : QTNH_EINVAL; }
 
-// Make the context's device current for the calling thread.
-qtnh_status enter(qtnh_ctx_t ctx) {
-  if (!ctx) return QTNH_EINVAL;
-  return check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
-}
+qtnh_status enter(qtnh_ctx_t ctx) { return enter_ctx(ctx); }
 }  // namespace
 
 extern "C" {
@@ -288,6 +291,8 @@ qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->desc) (void)hipFree(ctx->desc);
+  if (ctx->red) (void)hipFree(ctx->red);
   if (ctx->dev_scalar) (void)hipFree(ctx->dev_scalar);
   if (ctx->host_scalar) (void)hipHostFree(ctx->host_scalar);
   delete ctx;

@@ -1,0 +1,763 @@
+// Gate sequences on resident state vectors: qubit circuits as chains of the
+// in-place gate and diagonal contractions of the contraction path.
+//
+// The reference applies a circuit as one Tensor::contract per gate against
+// the state (examples/qft.cpp; src/tensor/contract.cpp:250-289 places each
+// gate's outputs on its inputs' axes).  Here a circuit is a list of ops, each
+// a dense k-qubit gate G[in][out] or a diagonal d[x], with qubit q = axis q of
+// a rank-n state with every dim 2 (axis 0 is the most significant index bit).
+//
+// Two executors:
+//  * per gate: one launch of the streaming gate / diagonal kernels per op;
+//  * fused: consecutive ops are grouped into runs touching at most `tb`
+//    distinct index bits.  A run is one pass over the state: each workgroup
+//    gathers a tile of 2^tb amplitudes (the run's bits vary, every other bit
+//    is fixed) into LDS, applies the run's ops in order, and scatters the tile
+//    back.  Every op does the same arithmetic on the same values as the per
+//    gate kernel (sum over inputs in row-major order from zero, products
+//    without FMA), so both executors agree bit for bit; the fused one reads
+//    and writes the state once per run instead of once per op.
+// Batches run the k-th run of many circuits (on many states) in one launch.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace qtnh_hip {
+
+qtnh_status launch_gate(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k, const uint16_t *axes,
+                        const double *G);
+qtnh_status launch_diag(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k, const uint16_t *axes,
+                        const double *d);
+
+namespace {
+
+typedef double double2_t __attribute__((ext_vector_type(2)));
+
+constexpr int kCircThreads = 256;
+constexpr int kMaxTileBits = 13;  // 2^13 amplitudes = 128 KiB of LDS
+constexpr int kMaxOpQubits = 3;
+
+struct TileOp {
+  int kind;                 // 0: dense gate, 1: diagonal
+  int kb;                   // qubits of the op
+  int pos[kMaxOpQubits];    // tile bit of op-index bit j (j = 0 most significant)
+  int zpos[kMaxOpQubits];   // the same tile bits, ascending
+  uint32_t coef;            // first coefficient (complex units)
+  uint32_t pad;
+};
+
+struct RunDesc {
+  const TileOp *ops;
+  const double2_t *coef;
+  double2_t *psi;
+  uint64_t first;       // first tile of this run in the launch's tile numbering
+  uint64_t outer_mask;  // state bits outside the tile
+  uint64_t lo_mask;     // the tile's lowest min(tb, 8) state bits
+  uint64_t hi_mask;     // the tile's other state bits
+  uint32_t op_begin, op_end;
+  int tb;               // tile bits
+  int n_outer;          // popcount(outer_mask)
+};
+
+// Parallel bit deposit: the low bits of v, in order, onto the set bits of mask.
+__device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask) {
+  uint64_t r = 0;
+  while (mask) {
+    const uint64_t low = mask & (0 - mask);
+    if (v & 1ull) r |= low;
+    v >>= 1;
+    mask ^= low;
+  }
+  return r;
+}
+
+__device__ __forceinline__ double2_t cmul2(double2_t a, double2_t b) {
+  double r, i;
+  cmul_parts(a.x, a.y, b.x, b.y, &r, &i);
+  return double2_t{r, i};
+}
+
+template <int KB>
+__device__ void gate_in_lds(double2_t *buf, int tb, const TileOp &op, const double2_t *g) {
+  constexpr int K = 1 << KB;
+  int comb[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    int off = 0;
+#pragma unroll
+    for (int j = 0; j < KB; ++j)
+      if ((c >> (KB - 1 - j)) & 1) off |= 1 << op.pos[j];
+    comb[c] = off;
+  }
+  const int groups = 1 << (tb - KB);
+  for (int gi = threadIdx.x; gi < groups; gi += kCircThreads) {
+    // 3-qubit gates: re-read the 64 coefficients from LDS in every group
+    // rather than holding them in 256 registers (a compiler barrier, no code).
+    if (KB == 3) asm volatile("" ::: "memory");
+    int base = gi;
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int p = op.zpos[j];
+      base = ((base >> p) << (p + 1)) | (base & ((1 << p) - 1));
+    }
+    double2_t in[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) in[c] = buf[base + comb[c]];
+#pragma unroll
+    for (int o = 0; o < K; ++o) {
+      double2_t acc = {0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const double2_t p = cmul2(in[i], g[i * K + o]);
+        acc.x = acc.x + p.x;
+        acc.y = acc.y + p.y;
+      }
+      buf[base + comb[o]] = acc;
+    }
+  }
+}
+
+template <int KB>
+__device__ void diag_in_lds(double2_t *buf, int tb, const TileOp &op, const double2_t *d) {
+  const int T = 1 << tb;
+  for (int l = threadIdx.x; l < T; l += kCircThreads) {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < KB; ++j) c = (c << 1) | ((l >> op.pos[j]) & 1);
+    buf[l] = cmul2(buf[l], d[c]);
+  }
+}
+
+// One workgroup per tile (persistent over tiles); runs[] sorted by `first`.
+template <int TBMAX>
+__global__ void __launch_bounds__(kCircThreads, (TBMAX >= 13 ? 1 : 2)) circuit_tile_kernel(const RunDesc *__restrict__ runs, int nruns,
+                                                                    uint64_t total_tiles) {
+  __shared__ double2_t buf[1 << TBMAX];
+  __shared__ double2_t gco[1 << (2 * kMaxOpQubits)];  // coefficients of the current op
+  for (uint64_t gt = blockIdx.x; gt < total_tiles; gt += gridDim.x) {
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (runs[mid].first <= gt)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const RunDesc &R = runs[lo];
+    const uint64_t t = gt - R.first;
+    const int T = 1 << R.tb;
+    // Tile element l = thread + kCircThreads j: its low 8 tile bits are the
+    // thread's, the others come from j.
+    double2_t *psi = R.psi + deposit(t, R.outer_mask);
+    const uint64_t lo_off = deposit(threadIdx.x, R.lo_mask);
+    for (int l = threadIdx.x, j = 0; l < T; l += kCircThreads, ++j) buf[l] = psi[lo_off | deposit(j, R.hi_mask)];
+    __syncthreads();
+    for (uint32_t o = R.op_begin; o < R.op_end; ++o) {
+      const TileOp op = R.ops[o];
+      const int nco = op.kind == 1 ? (1 << op.kb) : (1 << (2 * op.kb));
+      if ((int)threadIdx.x < nco) gco[threadIdx.x] = R.coef[op.coef + threadIdx.x];
+      __syncthreads();
+      if (op.kind == 1) {
+        if (op.kb == 1)
+          diag_in_lds<1>(buf, R.tb, op, gco);
+        else if (op.kb == 2)
+          diag_in_lds<2>(buf, R.tb, op, gco);
+        else
+          diag_in_lds<3>(buf, R.tb, op, gco);
+      } else if (op.kb == 1)
+        gate_in_lds<1>(buf, R.tb, op, gco);
+      else if (op.kb == 2)
+        gate_in_lds<2>(buf, R.tb, op, gco);
+      else
+        gate_in_lds<3>(buf, R.tb, op, gco);
+      __syncthreads();
+    }
+    for (int l = threadIdx.x, j = 0; l < T; l += kCircThreads, ++j) psi[lo_off | deposit(j, R.hi_mask)] = buf[l];
+    __syncthreads();
+  }
+}
+
+// ---- <Z_q> for a list of qubits --------------------------------------------
+// Chunks of 2^kChunkBits consecutive amplitudes.  Per chunk: the total
+// probability and, for each of the chunk's own index bits, the probability
+// with that bit set.  Bits above the chunk are constant inside it.
+constexpr int kChunkBits = 12;
+constexpr int kZThreads = 256;
+constexpr int kZSlots = 1 + kChunkBits;
+
+__global__ void __launch_bounds__(kZThreads) zchunk_kernel(const double2_t *__restrict__ psi, uint64_t n, int cb,
+                                                           double *__restrict__ part, uint64_t nchunks) {
+  __shared__ double red[kZThreads];
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    // Thread t owns chunk elements t + kZThreads j: bits 0..7 of the element are t's bits.
+    double tot = 0.0, hib[kChunkBits];
+#pragma unroll
+    for (int b = 0; b < kChunkBits; ++b) hib[b] = 0.0;
+    const uint64_t e0 = ch << cb;
+    const int csize = 1 << cb;
+    for (int l = threadIdx.x; l < csize; l += kZThreads) {
+      const uint64_t e = e0 + l;
+      double p = 0.0;
+      if (e < n) {
+        const double2_t v = psi[e];
+        p = v.x * v.x;
+        p = p + v.y * v.y;
+      }
+      tot = tot + p;
+#pragma unroll
+      for (int b = 8; b < kChunkBits; ++b)
+        if ((l >> b) & 1) hib[b] = hib[b] + p;
+    }
+    // slot 0: total; slot 1 + b: probability with bit b set.
+#pragma unroll
+    for (int s = 0; s < kZSlots; ++s) {
+      double v;
+      if (s == 0)
+        v = tot;
+      else if (s - 1 < 8)
+        v = ((threadIdx.x >> (s - 1)) & 1) ? tot : 0.0;
+      else
+        v = hib[s - 1];
+      red[threadIdx.x] = v;
+      __syncthreads();
+      for (int w = kZThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) part[ch * kZSlots + s] = red[0];
+      __syncthreads();
+    }
+  }
+}
+
+// One workgroup per requested qubit: <Z_b> = sum over chunks of (total - 2 set_b).
+__global__ void __launch_bounds__(kZThreads) zfinal_kernel(const double *__restrict__ part, uint64_t nchunks, int cb,
+                                                           const int *__restrict__ bits, double *__restrict__ out) {
+  __shared__ double red[kZThreads];
+  const int b = bits[blockIdx.x];
+  double s = 0.0;
+  for (uint64_t ch = threadIdx.x; ch < nchunks; ch += kZThreads) {
+    const double tot = part[ch * kZSlots];
+    double v;
+    if (b < cb)
+      v = tot - 2.0 * part[ch * kZSlots + 1 + b];
+    else
+      v = (((ch << cb) >> b) & 1) ? -tot : tot;
+    s = s + v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kZThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+}  // namespace
+}  // namespace qtnh_hip
+
+using qtnh_hip::c128;
+using qtnh_hip::RunDesc;
+using qtnh_hip::TileOp;
+
+// ---- host side ---------------------------------------------------------------
+struct qtnh_circuit_s {
+  struct Op {
+    int kind;  // 0 gate, 1 diagonal
+    int k;
+    uint16_t q[3];
+    std::vector<double> coef;  // interleaved complex: gates (2^k)^2, diagonals 2^k
+  };
+  int n = 0;
+  std::vector<Op> ops;
+  // Compiled form (device), rebuilt when ops or the tile size change.
+  int compiled_tb = -1;
+  int device = -1;
+  size_t compiled_ops = 0;
+  TileOp *d_ops = nullptr;
+  qtnh_hip::double2_t *d_coef = nullptr;
+  struct Run {
+    uint32_t begin, end;
+    std::vector<int> tbits;  // ascending state bits
+  };
+  std::vector<Run> runs;
+};
+
+namespace qtnh_hip {
+namespace {
+
+int g_tile_bits = 12;
+
+uint64_t mask_of(const qtnh_circuit_s &c, const qtnh_circuit_s::Op &op) {
+  uint64_t m = 0;
+  for (int j = 0; j < op.k; ++j) m |= 1ull << (c.n - 1 - op.q[j]);
+  return m;
+}
+
+// Greedy partition of the op sequence into runs of at most tb distinct bits,
+// each widened with the lowest free bits (coalesced tile rows) up to tb.
+void plan_runs(qtnh_circuit_s &c, int tb) {
+  c.runs.clear();
+  const int tbe = std::min(tb, c.n);
+  uint64_t cur = 0;
+  uint32_t begin = 0;
+  auto close = [&](uint32_t end) {
+    if (end == begin) return;
+    uint64_t m = cur;
+    for (int b = 0; b < c.n && __builtin_popcountll(m) < tbe; ++b) m |= 1ull << b;
+    qtnh_circuit_s::Run r;
+    r.begin = begin;
+    r.end = end;
+    for (int b = 0; b < c.n; ++b)
+      if ((m >> b) & 1) r.tbits.push_back(b);
+    c.runs.push_back(r);
+  };
+  for (uint32_t i = 0; i < c.ops.size(); ++i) {
+    const uint64_t m = mask_of(c, c.ops[i]);
+    if (__builtin_popcountll(cur | m) > tbe) {
+      close(i);
+      begin = i;
+      cur = 0;
+    }
+    cur |= m;
+  }
+  close((uint32_t)c.ops.size());
+}
+
+void free_compiled(qtnh_circuit_s &c) {
+  if (c.d_ops) (void)hipFree(c.d_ops);
+  if (c.d_coef) (void)hipFree(c.d_coef);
+  c.d_ops = nullptr;
+  c.d_coef = nullptr;
+  c.compiled_tb = -1;
+}
+
+qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
+  if (c.compiled_tb == g_tile_bits && c.device == ctx->device && c.compiled_ops == c.ops.size()) return QTNH_OK;
+  if (c.d_ops || c.d_coef) {
+    qtnh_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "circuit recompile sync");
+    if (st) return st;
+    free_compiled(c);
+  }
+  plan_runs(c, g_tile_bits);
+  std::vector<TileOp> tops(std::max<size_t>(c.ops.size(), 1));
+  std::vector<double> coef;
+  for (const auto &r : c.runs) {
+    for (uint32_t i = r.begin; i < r.end; ++i) {
+      const auto &op = c.ops[i];
+      TileOp t;
+      memset(&t, 0, sizeof(t));
+      t.kind = op.kind;
+      t.kb = op.k;
+      for (int j = 0; j < op.k; ++j) {
+        const int sb = c.n - 1 - op.q[j];
+        t.pos[j] = (int)(std::find(r.tbits.begin(), r.tbits.end(), sb) - r.tbits.begin());
+        t.zpos[j] = t.pos[j];
+      }
+      std::sort(t.zpos, t.zpos + op.k);
+      t.coef = (uint32_t)(coef.size() / 2);
+      coef.insert(coef.end(), op.coef.begin(), op.coef.end());
+      tops[i] = t;
+    }
+  }
+  if (coef.empty()) coef.assign(2, 0.0);
+  qtnh_status st = check_hip(ctx, hipMalloc((void **)&c.d_ops, tops.size() * sizeof(TileOp)), "circuit alloc");
+  if (!st) st = check_hip(ctx, hipMalloc((void **)&c.d_coef, coef.size() * sizeof(double)), "circuit alloc");
+  if (!st)
+    st = check_hip(ctx, hipMemcpyAsync(c.d_ops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice,
+                                       ctx->stream),
+                   "circuit upload");
+  if (!st)
+    st = check_hip(ctx, hipMemcpyAsync(c.d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
+                                       ctx->stream),
+                   "circuit upload");
+  if (st) {
+    free_compiled(c);
+    return st;
+  }
+  c.compiled_tb = g_tile_bits;
+  c.device = ctx->device;
+  c.compiled_ops = c.ops.size();
+  return QTNH_OK;
+}
+
+RunDesc make_desc(const qtnh_circuit_s &c, const qtnh_circuit_s::Run &r, c128 *psi, uint64_t first) {
+  RunDesc d;
+  memset(&d, 0, sizeof(d));
+  d.ops = c.d_ops;
+  d.coef = c.d_coef;
+  d.psi = (double2_t *)psi;
+  d.first = first;
+  d.op_begin = r.begin;
+  d.op_end = r.end;
+  d.tb = (int)r.tbits.size();
+  uint64_t tile = 0;
+  for (size_t j = 0; j < r.tbits.size(); ++j) {
+    tile |= 1ull << r.tbits[j];
+    if (j < 8)
+      d.lo_mask |= 1ull << r.tbits[j];
+    else
+      d.hi_mask |= 1ull << r.tbits[j];
+  }
+  const uint64_t all = c.n == 64 ? ~0ull : ((1ull << c.n) - 1);
+  d.outer_mask = all & ~tile;
+  d.n_outer = __builtin_popcountll(d.outer_mask);
+  return d;
+}
+
+uint64_t desc_tiles(const RunDesc &d) { return 1ull << d.n_outer; }
+
+// Launches groups of runs: group g covers descs [starts[g], starts[g+1]), one
+// launch each, in order.  The descriptors go to the device in one copy.
+qtnh_status launch_run_groups(qtnh_ctx_t ctx, const std::vector<RunDesc> &descs, const std::vector<size_t> &starts) {
+  if (descs.empty()) return QTNH_OK;
+  const uint64_t bytes = descs.size() * sizeof(RunDesc);
+  qtnh_status st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, bytes);
+  if (st) return st;
+  st = check_hip(ctx, hipMemcpyAsync(ctx->desc, descs.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+                 "run descriptor upload");
+  if (st) return st;
+  const RunDesc *dd = (const RunDesc *)ctx->desc;
+  for (size_t g = 0; g + 1 < starts.size(); ++g) {
+    const size_t b = starts[g], e = starts[g + 1];
+    if (b == e) continue;
+    int tbmax = 0;
+    uint64_t total = 0;
+    for (size_t i = b; i < e; ++i) {
+      tbmax = std::max(tbmax, descs[i].tb);
+      total = descs[i].first + desc_tiles(descs[i]);
+    }
+    const unsigned per_cu = tbmax <= 10 ? 8 : tbmax == 11 ? 4 : tbmax == 12 ? 2 : 1;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * per_cu);
+    const int nr = (int)(e - b);
+    if (tbmax <= 10)
+      hipLaunchKernelGGL(circuit_tile_kernel<10>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
+    else if (tbmax == 11)
+      hipLaunchKernelGGL(circuit_tile_kernel<11>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
+    else if (tbmax == 12)
+      hipLaunchKernelGGL(circuit_tile_kernel<12>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
+    else
+      hipLaunchKernelGGL(circuit_tile_kernel<13>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
+    st = check_hip(ctx, hipGetLastError(), "circuit tile launch");
+    if (st) return st;
+  }
+  return QTNH_OK;
+}
+
+qtnh_status run_fused(qtnh_ctx_t ctx, qtnh_circuit_s &c, c128 *psi) {
+  qtnh_status st = compile(ctx, c);
+  if (st) return st;
+  std::vector<RunDesc> descs;
+  std::vector<size_t> starts;
+  for (const auto &r : c.runs) {
+    starts.push_back(descs.size());
+    descs.push_back(make_desc(c, r, psi, 0));
+  }
+  starts.push_back(descs.size());
+  return launch_run_groups(ctx, descs, starts);
+}
+
+qtnh_status run_per_gate(qtnh_ctx_t ctx, const qtnh_circuit_s &c, c128 *psi) {
+  std::vector<uint64_t> dims(c.n, 2);
+  for (const auto &op : c.ops) {
+    qtnh_status st = op.kind == 0 ? launch_gate(ctx, psi, c.n, dims.data(), op.k, op.q, op.coef.data())
+                                  : launch_diag(ctx, psi, c.n, dims.data(), op.k, op.q, op.coef.data());
+    if (st) return st;
+  }
+  return QTNH_OK;
+}
+
+// ---- seeded generators ----------------------------------------------------
+uint64_t mix64(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct HostRng {
+  uint64_t seed, k = 0;
+  explicit HostRng(uint64_t s) : seed(s) {}
+  uint64_t next() { return mix64(seed, k++); }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }  // [0, 1)
+  void normal2(double *a, double *b) {
+    const double u1 = (double)((next() >> 11) + 1) * 0x1.0p-53, u2 = uniform();
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    *a = r * std::cos(2.0 * M_PI * u2);
+    *b = r * std::sin(2.0 * M_PI * u2);
+  }
+};
+
+// Haar-random SU(2) (Mezzadri 2007): QR of a complex Gaussian 2x2 with the
+// phases of R's diagonal moved into Q, then divided by sqrt(det).
+// Returns G[in][out] = U[out][in].
+void random_su2(HostRng &rng, double *G) {
+  double z[4][2];
+  for (int i = 0; i < 4; ++i) rng.normal2(&z[i][0], &z[i][1]);
+  // columns a = (z00, z10), b = (z01, z11) of Z
+  const double ar[2] = {z[0][0], z[2][0]}, ai[2] = {z[0][1], z[2][1]};
+  const double br[2] = {z[1][0], z[3][0]}, bi[2] = {z[1][1], z[3][1]};
+  const double na = std::sqrt(ar[0] * ar[0] + ai[0] * ai[0] + ar[1] * ar[1] + ai[1] * ai[1]);
+  double q0r[2], q0i[2];
+  for (int i = 0; i < 2; ++i) {
+    q0r[i] = ar[i] / na;
+    q0i[i] = ai[i] / na;
+  }
+  // r01 = <q0, b>; b' = b - q0 r01
+  double r01r = 0, r01i = 0;
+  for (int i = 0; i < 2; ++i) {
+    r01r += q0r[i] * br[i] + q0i[i] * bi[i];
+    r01i += q0r[i] * bi[i] - q0i[i] * br[i];
+  }
+  double cr[2], ci[2];
+  for (int i = 0; i < 2; ++i) {
+    cr[i] = br[i] - (q0r[i] * r01r - q0i[i] * r01i);
+    ci[i] = bi[i] - (q0r[i] * r01i + q0i[i] * r01r);
+  }
+  const double nc = std::sqrt(cr[0] * cr[0] + ci[0] * ci[0] + cr[1] * cr[1] + ci[1] * ci[1]);
+  double q1r[2], q1i[2];
+  for (int i = 0; i < 2; ++i) {
+    q1r[i] = cr[i] / nc;
+    q1i[i] = ci[i] / nc;
+  }
+  // R's diagonal (na, nc) is real positive, so Q is already phase-fixed.
+  // U = Q / sqrt(det Q)
+  const double dr = (q0r[0] * q1r[1] - q0i[0] * q1i[1]) - (q1r[0] * q0r[1] - q1i[0] * q0i[1]);
+  const double di = (q0r[0] * q1i[1] + q0i[0] * q1r[1]) - (q1r[0] * q0i[1] + q1i[0] * q0r[1]);
+  const double ph = std::atan2(di, dr) / 2.0;
+  const double sr = std::cos(-ph), si = std::sin(-ph);
+  double U[2][2][2];  // U[row][col][re/im]
+  for (int i = 0; i < 2; ++i) {
+    U[i][0][0] = q0r[i] * sr - q0i[i] * si;
+    U[i][0][1] = q0r[i] * si + q0i[i] * sr;
+    U[i][1][0] = q1r[i] * sr - q1i[i] * si;
+    U[i][1][1] = q1r[i] * si + q1i[i] * sr;
+  }
+  for (int in = 0; in < 2; ++in)
+    for (int out = 0; out < 2; ++out) {
+      G[2 * (in * 2 + out)] = U[out][in][0];
+      G[2 * (in * 2 + out) + 1] = U[out][in][1];
+    }
+}
+
+bool qubits_valid(int n, int k, const uint16_t *q) {
+  if (k < 1 || k > kMaxOpQubits) return false;
+  for (int j = 0; j < k; ++j) {
+    if (q[j] >= n) return false;
+    for (int i = 0; i < j; ++i)
+      if (q[i] == q[j]) return false;
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace qtnh_hip
+
+using namespace qtnh_hip;
+
+extern "C" {
+
+void qtnh_tune_circuit(int tile_bits) { g_tile_bits = std::max(3, std::min(tile_bits, kMaxTileBits)); }
+
+qtnh_status qtnh_circuit_create(int n_qubits, qtnh_circuit_t *out) {
+  if (!out) return QTNH_EINVAL;
+  *out = nullptr;
+  if (n_qubits < 1 || n_qubits > 62) return QTNH_EINVAL;
+  auto *c = new (std::nothrow) qtnh_circuit_s;
+  if (!c) return QTNH_ENOMEM;
+  c->n = n_qubits;
+  *out = c;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_destroy(qtnh_circuit_t c) {
+  if (!c) return QTNH_EINVAL;
+  if (c->device >= 0) {
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+  }
+  free_compiled(*c);
+  delete c;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_gate(qtnh_circuit_t c, int k, const uint16_t *qubits, const double *G) {
+  if (!c || !qubits || !G || !qubits_valid(c->n, k, qubits)) return QTNH_EINVAL;
+  qtnh_circuit_s::Op op;
+  op.kind = 0;
+  op.k = k;
+  for (int j = 0; j < k; ++j) op.q[j] = qubits[j];
+  op.coef.assign(G, G + 2 * (1 << k) * (1 << k));
+  c->ops.push_back(std::move(op));
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_diag(qtnh_circuit_t c, int k, const uint16_t *qubits, const double *d) {
+  if (!c || !qubits || !d || !qubits_valid(c->n, k, qubits)) return QTNH_EINVAL;
+  qtnh_circuit_s::Op op;
+  op.kind = 1;
+  op.k = k;
+  for (int j = 0; j < k; ++j) op.q[j] = qubits[j];
+  op.coef.assign(d, d + 2 * (1 << k));
+  c->ops.push_back(std::move(op));
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_qft(qtnh_circuit_t c, const double *phases) {
+  if (!c) return QTNH_EINVAL;
+  const int n = c->n;
+  const double h = 0x1.6a09e667f3bcdp-1;
+  const double H[8] = {h, 0, h, 0, h, 0, -h, 0};
+  for (int i = 0; i < n; ++i) {
+    const uint16_t qi = (uint16_t)i;
+    qtnh_status st = qtnh_circuit_gate(c, 1, &qi, H);
+    if (st) return st;
+    for (int j = i + 1; j < n; ++j) {
+      const int m = j - i;
+      double pr, pi;
+      if (phases) {
+        pr = phases[2 * m];
+        pi = phases[2 * m + 1];
+      } else {
+        pr = std::cos(M_PI / std::ldexp(1.0, m));
+        pi = std::sin(M_PI / std::ldexp(1.0, m));
+      }
+      const double d[8] = {1, 0, 1, 0, 1, 0, pr, pi};
+      const uint16_t q2[2] = {(uint16_t)i, (uint16_t)j};
+      st = qtnh_circuit_diag(c, 2, q2, d);
+      if (st) return st;
+    }
+  }
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_random(qtnh_circuit_t c, int depth, uint64_t seed) {
+  if (!c || depth < 0) return QTNH_EINVAL;
+  const int n = c->n;
+  HostRng rng(seed);
+  std::vector<uint16_t> perm(n);
+  for (int layer = 0; layer < depth; ++layer) {
+    for (int q = 0; q < n; ++q) {
+      double G[8];
+      random_su2(rng, G);
+      const uint16_t qq = (uint16_t)q;
+      qtnh_status st = qtnh_circuit_gate(c, 1, &qq, G);
+      if (st) return st;
+    }
+    for (int q = 0; q < n; ++q) perm[q] = (uint16_t)q;
+    for (int q = n - 1; q > 0; --q) std::swap(perm[q], perm[rng.next() % (uint64_t)(q + 1)]);
+    const double cz[8] = {1, 0, 1, 0, 1, 0, -1, 0};
+    for (int p = 0; p + 1 < n; p += 2) {
+      const uint16_t q2[2] = {perm[p], perm[p + 1]};
+      qtnh_status st = qtnh_circuit_diag(c, 2, q2, cz);
+      if (st) return st;
+    }
+  }
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, uint64_t *n_runs) {
+  if (!c) return QTNH_EINVAL;
+  if (n_qubits) *n_qubits = c->n;
+  if (n_ops) *n_ops = c->ops.size();
+  if (n_runs) {
+    qtnh_circuit_s tmp;
+    tmp.n = c->n;
+    tmp.ops = c->ops;
+    plan_runs(tmp, g_tile_bits);
+    *n_runs = tmp.runs.size();
+  }
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_op(qtnh_circuit_t c, uint64_t i, int *kind, int *k, uint16_t *qubits, double *coef) {
+  if (!c || i >= c->ops.size()) return QTNH_EINVAL;
+  const auto &op = c->ops[i];
+  if (kind) *kind = op.kind;
+  if (k) *k = op.k;
+  if (qubits)
+    for (int j = 0; j < op.k; ++j) qubits[j] = op.q[j];
+  if (coef) memcpy(coef, op.coef.data(), op.coef.size() * sizeof(double));
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_circuit_run(qtnh_ctx_t ctx, qtnh_circuit_t c, void *state, int mode) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!c || !state) return set_error(ctx, QTNH_EINVAL, "circuit_run: null argument");
+  if (mode == QTNH_RUN_PER_GATE) return run_per_gate(ctx, *c, (c128 *)state);
+  if (mode == QTNH_RUN_FUSED) return run_fused(ctx, *c, (c128 *)state);
+  return set_error(ctx, QTNH_EINVAL, "circuit_run: unknown mode %d", mode);
+}
+
+qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit_t *cs, void *const *states) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (count < 0 || (count > 0 && (!cs || !states))) return set_error(ctx, QTNH_EINVAL, "run_batch: bad arguments");
+  size_t steps = 0;
+  for (int i = 0; i < count; ++i) {
+    if (!cs[i] || !states[i]) return set_error(ctx, QTNH_EINVAL, "run_batch: null circuit or state %d", i);
+    for (int j = 0; j < i; ++j)
+      if (states[j] == states[i]) return set_error(ctx, QTNH_EINVAL, "run_batch: state %d repeats state %d", i, j);
+    st = compile(ctx, *cs[i]);
+    if (st) return st;
+    steps = std::max(steps, cs[i]->runs.size());
+  }
+  // Step s runs the s-th run of every circuit that has one, in one launch.
+  std::vector<RunDesc> descs;
+  std::vector<size_t> starts;
+  for (size_t s = 0; s < steps; ++s) {
+    starts.push_back(descs.size());
+    uint64_t first = 0;
+    for (int i = 0; i < count; ++i) {
+      if (s >= cs[i]->runs.size()) continue;
+      RunDesc d = make_desc(*cs[i], cs[i]->runs[s], (c128 *)states[i], first);
+      first += desc_tiles(d);
+      descs.push_back(d);
+    }
+  }
+  starts.push_back(descs.size());
+  return launch_run_groups(ctx, descs, starts);
+}
+
+qtnh_status qtnh_expect_z_c128(qtnh_ctx_t ctx, const void *state, int n_qubits, int nq, const uint16_t *qubits,
+                               double *out) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!state || !out || n_qubits < 1 || n_qubits > 62 || nq < 1 || nq > 1024 || !qubits)
+    return set_error(ctx, QTNH_EINVAL, "expect_z: bad arguments");
+  for (int j = 0; j < nq; ++j)
+    if (qubits[j] >= n_qubits) return set_error(ctx, QTNH_EINVAL, "expect_z: qubit %d out of range", qubits[j]);
+  const uint64_t n = 1ull << n_qubits;
+  const int cb = std::min(n_qubits, kChunkBits);
+  const uint64_t nchunks = n >> cb;
+  const uint64_t part_bytes = ((nchunks * kZSlots * sizeof(double)) + 255) & ~255ull;
+  st = grow_buffer(ctx, &ctx->red, &ctx->red_bytes, part_bytes + (uint64_t)nq * (sizeof(int) + sizeof(double)) + 512);
+  if (st) return st;
+  double *part = (double *)ctx->red;
+  int *bits = (int *)((char *)ctx->red + part_bytes);
+  double *dout = (double *)((char *)ctx->red + part_bytes + (((uint64_t)nq * sizeof(int) + 255) & ~255ull));
+  std::vector<int> hb(nq);
+  for (int j = 0; j < nq; ++j) hb[j] = n_qubits - 1 - qubits[j];
+  st = check_hip(ctx, hipMemcpyAsync(bits, hb.data(), nq * sizeof(int), hipMemcpyHostToDevice, ctx->stream),
+                 "expect_z upload");
+  if (st) return st;
+  const unsigned blocks = (unsigned)std::min<uint64_t>(nchunks, (uint64_t)ctx->num_cus * 8);
+  hipLaunchKernelGGL(zchunk_kernel, dim3(blocks), dim3(kZThreads), 0, ctx->stream, (const double2_t *)state, n, cb,
+                     part, nchunks);
+  st = check_hip(ctx, hipGetLastError(), "expect_z launch");
+  if (st) return st;
+  hipLaunchKernelGGL(zfinal_kernel, dim3(nq), dim3(kZThreads), 0, ctx->stream, part, nchunks, cb, bits, dout);
+  st = check_hip(ctx, hipGetLastError(), "expect_z final launch");
+  if (st) return st;
+  st = check_hip(ctx, hipMemcpyAsync(out, dout, nq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                 "expect_z copy");
+  if (st) return st;
+  return check_hip(ctx, hipStreamSynchronize(ctx->stream), "expect_z sync");
+}
+
+}  // extern "C"

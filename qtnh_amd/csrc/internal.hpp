@@ -21,6 +21,12 @@ struct qtnh_ctx_s {
   // Small pinned host buffer + device buffer for reduction results.
   double *host_scalar = nullptr;
   double *dev_scalar = nullptr;
+  // Growable device buffer for launch descriptors (circuit runs, batches).
+  void *desc = nullptr;
+  uint64_t desc_bytes = 0;
+  // Growable device buffer for reduction partials (expectation values).
+  void *red = nullptr;
+  uint64_t red_bytes = 0;
 };
 
 namespace qtnh_hip {
@@ -36,6 +42,18 @@ qtnh_status set_error(qtnh_ctx_t ctx, qtnh_status st, const char *fmt, ...);
 qtnh_status check_hip(qtnh_ctx_t ctx, hipError_t e, const char *what);
 // Ensures ctx->ws holds at least `bytes`; returns its pointer in *out.
 qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out);
+// Grows a context-owned device buffer (*buf, *cap) to at least `bytes`; the old
+// buffer is freed only after the stream has drained.
+qtnh_status grow_buffer(qtnh_ctx_t ctx, void **buf, uint64_t *cap, uint64_t bytes);
+// Makes the context's device current for the calling thread.
+qtnh_status enter_ctx(qtnh_ctx_t ctx);
+
+// Complex product without FMA contraction (files are built with -ffp-contract=off):
+// the arithmetic of std::complex<double> operator* in a GCC build without -march.
+__host__ __device__ inline void cmul_parts(double ar, double ai, double br, double bi, double *r, double *i) {
+  *r = ar * br - ai * bi;
+  *i = ar * bi + ai * br;
+}
 
 // Row-major size with overflow detection; false on overflow.
 bool checked_size(int rank, const uint64_t *dims, uint64_t *n);

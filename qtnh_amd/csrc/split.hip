@@ -28,8 +28,6 @@ namespace {
 typedef int int4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
-constexpr int OBM = 64, OBN = 64, OBK = 64;  // block tile (real rows, real cols, k bytes)
-constexpr int kRowPad = 80;                  // LDS bytes per 64-byte row: conflict-free fragment reads
 constexpr int kMaxSlices = 8;
 
 // One block per complex row.  mode 0 (A): real row i = (re_0, im_0, re_1, ...).
@@ -85,22 +83,41 @@ __global__ void __launch_bounds__(kThreads) split_rows_kernel(const c128 *__rest
   }
 }
 
+// Block tile: 128 real rows x 64 real columns, 8 waves of 32 x 32.  Operand
+// panels (S slices x rows x 64 bytes of depth) are staged through LDS, two
+// stages when they fit (S <= 6), with 16-byte chunks XOR-swizzled by row so
+// that the fragment reads of 16 rows hit distinct banks.
+constexpr int kGemmThreads = 512;
+constexpr int BM = 128, BN = 64, BK = 64;
+constexpr int kGroupM = 4;  // tile rows per group of the tile order
+
+__device__ __forceinline__ int lds_at(int p, int r, int c, int rows) { return ((p * rows + r) << 2) + (c ^ ((r >> 2) & 3)); }
+
 // C (M x N2 real, leading dim ldc in doubles) (+)= recombined slice products.
 // As: S planes of M x K2 int8 (row-major, ld lda); Bt: S planes of N2 x K2 int8.
 template <int S>
-__global__ void __launch_bounds__(kThreads) ozaki_gemm_kernel(const int8_t *__restrict__ As, uint64_t a_plane,
-                                                              uint64_t lda, const int8_t *__restrict__ Bt,
-                                                              uint64_t b_plane, uint64_t ldb, const int *__restrict__ ea,
-                                                              const int *__restrict__ eb, double *__restrict__ C,
-                                                              uint64_t ldc, uint64_t M, uint64_t N2, uint64_t K2,
-                                                              int accumulate) {
-  __shared__ int4_t sA[S * OBM * kRowPad / 16];
-  __shared__ int4_t sB[S * OBN * kRowPad / 16];
+__global__ void __launch_bounds__(kGemmThreads) ozaki_gemm_kernel(
+    const int8_t *__restrict__ As, uint64_t a_plane, uint64_t lda, const int8_t *__restrict__ Bt, uint64_t b_plane,
+    uint64_t ldb, const int *__restrict__ ea, const int *__restrict__ eb, double *__restrict__ C, uint64_t ldc,
+    uint64_t M, uint64_t N2, uint64_t K2, unsigned tilesM, unsigned tilesN, int accumulate) {
+  constexpr int NST = (S <= 6) ? 2 : 1;
+  __shared__ int4_t sA[NST][S * BM * 4];
+  __shared__ int4_t sB[NST][S * BN * 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const uint64_t ntn = (N2 + OBN - 1) / OBN;
-  const uint64_t m0 = ((uint64_t)blockIdx.x / ntn) * OBM;
-  const uint64_t n0 = ((uint64_t)blockIdx.x % ntn) * OBN;
+
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+  // range of tiles, walked in groups of kGroupM tile rows, so the blocks an XCD
+  // runs together share A and B panels in its L2.
+  const unsigned T = gridDim.x, b = blockIdx.x;
+  const unsigned xcd = b & 7, per = T >> 3, rem = T & 7;
+  const unsigned t = xcd * per + min(xcd, rem) + (b >> 3);
+  const unsigned gsize = kGroupM * tilesN;
+  const unsigned first_m = (t / gsize) * kGroupM;
+  const unsigned gm_cnt = min((unsigned)kGroupM, tilesM - first_m);
+  const unsigned in = t % gsize;
+  const uint64_t m0 = (uint64_t)(first_m + in % gm_cnt) * BM;
+  const uint64_t n0 = (uint64_t)(in / gm_cnt) * BN;
 
   int4_t acc[S][2][2];
 #pragma unroll
@@ -110,40 +127,57 @@ __global__ void __launch_bounds__(kThreads) ozaki_gemm_kernel(const int8_t *__re
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[d][i][j] = int4_t{0, 0, 0, 0};
 
-  // Staging: thread -> (row tid / 4, 16-byte chunk tid % 4) of each 64 x 64-byte plane tile.
-  const int sr = tid >> 2, sc = (tid & 3) * 16;
+  // Staging: A chunk (row tid / 4, 16-byte chunk tid % 4) of every slice; B
+  // chunks tid + 512 i over the S x 64 x 4 chunks of the B panel.
+  constexpr int BL = (S * BN * 4 + kGemmThreads - 1) / kGemmThreads;
   const int4_t zero = {0, 0, 0, 0};
-  int4_t pa[S], pb[S];
+  int4_t pa[S], pb[BL];
+  const int ar = tid >> 2, ac = tid & 3;
   auto fetch = [&](uint64_t k0) {
-    const uint64_t ga = m0 + sr, gb = n0 + sr, gk = k0 + sc;
-    // K2 is a multiple of 64 (the caller pads the real depth with zeros), so a chunk is all in or all out.
+    const uint64_t ga = m0 + ar;
 #pragma unroll
-    for (int p = 0; p < S; ++p) {
-      pa[p] = (ga < M && gk < K2) ? *reinterpret_cast<const int4_t *>(As + p * a_plane + ga * lda + gk) : zero;
-      pb[p] = (gb < N2 && gk < K2) ? *reinterpret_cast<const int4_t *>(Bt + p * b_plane + gb * ldb + gk) : zero;
+    for (int p = 0; p < S; ++p)
+      pa[p] = (ga < M) ? *reinterpret_cast<const int4_t *>(As + p * a_plane + ga * lda + k0 + ac * 16) : zero;
+#pragma unroll
+    for (int i = 0; i < BL; ++i) {
+      const int c = tid + kGemmThreads * i;
+      const int p = c / (BN * 4), r = (c % (BN * 4)) >> 2, ch = c & 3;
+      const uint64_t gb = n0 + r;
+      pb[i] = (c < S * BN * 4 && gb < N2)
+                  ? *reinterpret_cast<const int4_t *>(Bt + p * b_plane + gb * ldb + k0 + ch * 16)
+                  : zero;
     }
   };
-  fetch(0);
-  for (uint64_t k0 = 0; k0 < K2; k0 += OBK) {
+  auto stash = [&](int st) {
 #pragma unroll
-    for (int p = 0; p < S; ++p) {
-      sA[(p * OBM + sr) * (kRowPad / 16) + sc / 16] = pa[p];
-      sB[(p * OBN + sr) * (kRowPad / 16) + sc / 16] = pb[p];
+    for (int p = 0; p < S; ++p) sA[st][lds_at(p, ar, ac, BM)] = pa[p];
+#pragma unroll
+    for (int i = 0; i < BL; ++i) {
+      const int c = tid + kGemmThreads * i;
+      if (c < S * BN * 4) sB[st][lds_at(c / (BN * 4), (c % (BN * 4)) >> 2, c & 3, BN)] = pb[i];
     }
-    __syncthreads();
-    if (k0 + OBK < K2) fetch(k0 + OBK);
+  };
+
+  // The caller pads the real depth K2 to a multiple of BK with zeros.
+  const uint64_t nk = K2 / BK;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (uint64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (NST == 2) ? (int)(kt & 1) : 0;
+    if (kt + 1 < nk) fetch((kt + 1) * BK);  // in flight during the MFMAs below
     // Fragments (16x16x64 i8): lane l holds row / column l % 16, k bytes 16 (l / 16) .. +15.
     // All B slices stay in registers; A slices stream through, one at a time.
     int4_t fb[S][2];
 #pragma unroll
     for (int q = 0; q < S; ++q)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) fb[q][j] = sB[(q * OBN + wn + 16 * j + (lane & 15)) * (kRowPad / 16) + (lane >> 4)];
+      for (int j = 0; j < 2; ++j) fb[q][j] = sB[cur][lds_at(q, wn + 16 * j + (lane & 15), lane >> 4, BN)];
 #pragma unroll
     for (int p = 0; p < S; ++p) {
       int4_t fa[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) fa[i] = sA[(p * OBM + wm + 16 * i + (lane & 15)) * (kRowPad / 16) + (lane >> 4)];
+      for (int i = 0; i < 2; ++i) fa[i] = sA[cur][lds_at(p, wm + 16 * i + (lane & 15), lane >> 4, BM)];
 #pragma unroll
       for (int q = 0; q < S - p; ++q)
 #pragma unroll
@@ -152,7 +186,16 @@ __global__ void __launch_bounds__(kThreads) ozaki_gemm_kernel(const int8_t *__re
           for (int j = 0; j < 2; ++j)
             acc[p + q][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[q][j], acc[p + q][i][j], 0, 0, 0);
     }
-    __syncthreads();
+    if (NST == 2) {
+      if (kt + 1 < nk) stash(cur ^ 1);
+      __syncthreads();
+    } else {
+      __syncthreads();
+      if (kt + 1 < nk) {
+        stash(0);
+        __syncthreads();
+      }
+    }
   }
   // 16x16 i32 result layout: lane l holds column l % 16 of rows 4 (l / 16) + v.
 #pragma unroll
@@ -164,22 +207,22 @@ __global__ void __launch_bounds__(kThreads) ozaki_gemm_kernel(const int8_t *__re
         const uint64_t gm = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
         const uint64_t gn = n0 + wn + 16 * j + (lane & 15);
         if (gm < M && gn < N2) {
-          double s = 0.0;
+          double sum = 0.0;
 #pragma unroll
-          for (int d = S - 1; d >= 0; --d) s += ldexp((double)acc[d][i][j][v], -(12 + 7 * d));
-          s = ldexp(s, ea[gm] + eb[gn]);
+          for (int d = S - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -(12 + 7 * d));
+          sum = ldexp(sum, ea[gm] + eb[gn]);
           double *c = C + gm * ldc + gn;
-          *c = accumulate ? *c + s : s;
+          *c = accumulate ? *c + sum : sum;
         }
       }
 }
 
 template <int S>
-void launch_ozaki(hipStream_t st, unsigned blocks, const int8_t *As, uint64_t ap, uint64_t lda, const int8_t *Bt,
-                  uint64_t bp, uint64_t ldb, const int *ea, const int *eb, double *C, uint64_t ldc, uint64_t M,
-                  uint64_t N2, uint64_t K2, int accumulate) {
-  hipLaunchKernelGGL((ozaki_gemm_kernel<S>), dim3(blocks), dim3(kThreads), 0, st, As, ap, lda, Bt, bp, ldb, ea, eb, C,
-                     ldc, M, N2, K2, accumulate);
+void launch_ozaki(hipStream_t st, unsigned tilesM, unsigned tilesN, const int8_t *As, uint64_t ap, uint64_t lda,
+                  const int8_t *Bt, uint64_t bp, uint64_t ldb, const int *ea, const int *eb, double *C, uint64_t ldc,
+                  uint64_t M, uint64_t N2, uint64_t K2, int accumulate) {
+  hipLaunchKernelGGL((ozaki_gemm_kernel<S>), dim3(tilesM * tilesN), dim3(kGemmThreads), 0, st, As, ap, lda, Bt, bp, ldb,
+                     ea, eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
 }
 
 }  // namespace
@@ -225,16 +268,16 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
   if (st) return st;
   // int32 accumulators stay exact while (S pairs) * 64 * 64 * depth < 2^31: run depth chunks of 2^15.
   const uint64_t chunk = 32768;
-  const uint64_t tiles = ((N2 + OBN - 1) / OBN) * ((M + OBM - 1) / OBM);
-  if (tiles > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "split: grid too large");
+  const uint64_t tilesM = (M + BM - 1) / BM, tilesN = (N2 + BN - 1) / BN;
+  if (tilesM * tilesN > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "split: grid too large");
   for (uint64_t k0 = 0; k0 < K2; k0 += chunk) {
     const uint64_t kc = std::min(chunk, K2 - k0);
     const int accumulate = k0 > 0;
     switch (S) {
 #define QTNH_SPLIT_CASE(s)                                                                                            \
   case s:                                                                                                            \
-    launch_ozaki<s>(ctx->stream, (unsigned)tiles, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, (double *)C, N2, \
-                    M, N2, kc, accumulate);                                                                          \
+    launch_ozaki<s>(ctx->stream, (unsigned)tilesM, (unsigned)tilesN, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, \
+                    (double *)C, N2, M, N2, kc, accumulate);                                                         \
     break;
       QTNH_SPLIT_CASE(1)
       QTNH_SPLIT_CASE(2)
@@ -244,8 +287,8 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
       QTNH_SPLIT_CASE(6)
       QTNH_SPLIT_CASE(7)
       default:
-        launch_ozaki<8>(ctx->stream, (unsigned)tiles, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, (double *)C,
-                        N2, M, N2, kc, accumulate);
+        launch_ozaki<8>(ctx->stream, (unsigned)tilesM, (unsigned)tilesN, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea,
+                        eb, (double *)C, N2, M, N2, kc, accumulate);
         break;
 #undef QTNH_SPLIT_CASE
     }

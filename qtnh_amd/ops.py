@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PREC_FP64, PREC_SPLIT, X, check, lib
+from ._lib import PREC_FP64, PREC_SPLIT, RUN_FUSED, RUN_PER_GATE, X, check, lib
 
 _CTX = {}
 
@@ -226,3 +226,124 @@ def tune_zgemm(bn=0):
 def tune_split(slices=6):
     """Number of 7-bit slices of split-precision contractions (process-wide)."""
     lib().qtnh_tune_split(slices)
+
+
+def tune_circuit(tile_bits=12):
+    """Tile bits (3..13) of fused circuit runs (process-wide)."""
+    lib().qtnh_tune_circuit(tile_bits)
+
+
+class Circuit:
+    """A qubit circuit (gate list) applied in place to resident 2^n states.
+
+    Qubit q is axis q of the rank-n state (axis 0 most significant); gates are
+    G[in][out] contracted over `in`, as in the reference's SymmTensor gates."""
+
+    def __init__(self, n_qubits):
+        h = ctypes.c_void_p()
+        check(lib().qtnh_circuit_create(n_qubits, ctypes.byref(h)))
+        self._h = h
+        self.n = n_qubits
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().qtnh_circuit_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def gate(self, qubits, g):
+        q, qp = _u16(qubits)
+        gh, gp = _c128_host(g)
+        if gh.size != (1 << (2 * len(qubits))):
+            raise ValueError("gate needs %d elements" % (1 << (2 * len(qubits))))
+        check(lib().qtnh_circuit_gate(self._h, len(qubits), qp, gp))
+        return self
+
+    def diag(self, qubits, d):
+        q, qp = _u16(qubits)
+        dh, dp = _c128_host(d)
+        if dh.size != (1 << len(qubits)):
+            raise ValueError("diagonal needs %d elements" % (1 << len(qubits)))
+        check(lib().qtnh_circuit_diag(self._h, len(qubits), qp, dp))
+        return self
+
+    def qft(self, phases=None):
+        if phases is None:
+            check(lib().qtnh_circuit_qft(self._h, None))
+        else:
+            ph, pp = _c128_host(phases)
+            check(lib().qtnh_circuit_qft(self._h, pp))
+        return self
+
+    def random(self, depth, seed):
+        check(lib().qtnh_circuit_random(self._h, depth, seed))
+        return self
+
+    def info(self):
+        n = ctypes.c_int(0)
+        nops = ctypes.c_uint64(0)
+        nruns = ctypes.c_uint64(0)
+        check(lib().qtnh_circuit_info(self._h, ctypes.byref(n), ctypes.byref(nops), ctypes.byref(nruns)))
+        return n.value, nops.value, nruns.value
+
+    def ops(self):
+        """[(kind, qubits, coef)] with kind 0 for dense gates, 1 for diagonals."""
+        out = []
+        _, nops, _ = self.info()
+        for i in range(nops):
+            kind = ctypes.c_int(0)
+            k = ctypes.c_int(0)
+            q = np.zeros(3, dtype=np.uint16)
+            co = np.zeros(64, dtype=np.complex128)
+            check(lib().qtnh_circuit_op(self._h, i, ctypes.byref(kind), ctypes.byref(k),
+                                        q.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                                        co.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+            size = (1 << (2 * k.value)) if kind.value == 0 else (1 << k.value)
+            out.append((kind.value, [int(v) for v in q[:k.value]], co[:size].copy()))
+        return out
+
+    def run(self, state, mode=RUN_FUSED):
+        if state.numel() != (1 << self.n):
+            raise ValueError("state has %d amplitudes, circuit needs %d" % (state.numel(), 1 << self.n))
+        ctx = context(state.device.index)
+        check(lib().qtnh_circuit_run(ctx, self._h, _ptr(state), mode), ctx)
+        return state
+
+
+def run_batch(circuits, states):
+    """Apply circuits[i] to states[i] in place, one launch per run index."""
+    if len(circuits) != len(states):
+        raise ValueError("one state per circuit")
+    for c, s in zip(circuits, states):
+        if s.numel() != (1 << c.n):
+            raise ValueError("state size does not match its circuit")
+    if not circuits:
+        return states
+    ctx = context(states[0].device.index)
+    hs = (ctypes.c_void_p * len(circuits))(*[c.handle.value for c in circuits])
+    ps = (ctypes.c_void_p * len(states))(*[s.data_ptr() for s in states])
+    for s in states:
+        _ptr(s)
+    check(lib().qtnh_circuit_run_batch(ctx, len(circuits), hs, ps), ctx)
+    return states
+
+
+def expect_z(state, n_qubits, qubits):
+    """[<psi|Z_q|psi> for q in qubits] (host floats; blocks)."""
+    if state.numel() != (1 << n_qubits):
+        raise ValueError("state size does not match n_qubits")
+    ctx = context(state.device.index)
+    q, qp = _u16(qubits)
+    out = np.zeros(max(len(qubits), 1), dtype=np.float64)
+    check(lib().qtnh_expect_z_c128(ctx, _ptr(state), n_qubits, len(qubits), qp,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), ctx)
+    return [float(v) for v in out[:len(qubits)]]

@@ -1,0 +1,161 @@
+"""Circuits on resident states, through the C ABI, against the oracle.
+
+Both executors (one launch per gate, and fused tiles of up to 13 qubits in
+LDS) must equal the oracle's gate-by-gate application bit for bit: every op
+does the reference's arithmetic (sum over inputs in row-major order from
+zero, products without FMA) on the same values in the same order.
+<Z_q> is checked against numpy brute force; large states through norm
+preservation and fused == per-gate.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from qtnh_amd import _lib, ops
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    yield
+    ops.tune_circuit(12)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def rnd_state(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    return x / np.linalg.norm(x)
+
+
+def oracle_run(circuit, x):
+    x = np.array(x, dtype=np.complex128, copy=True)
+    dims = [2] * circuit.n
+    for kind, q, co in circuit.ops():
+        if kind == 0:
+            oracle.gate_apply(x, dims, q, co, inplace=True)
+        else:
+            oracle.diag_apply(x, dims, q, co, inplace=True)
+    return x
+
+
+@pytest.mark.parametrize("n", [1, 3, 8, 12, 14, 16])
+@pytest.mark.parametrize("mode", [_lib.RUN_FUSED, _lib.RUN_PER_GATE])
+def test_qft_bitexact(n, mode):
+    phases = np.exp(1j * np.pi / 2.0 ** np.arange(n + 1))
+    x = rnd_state(n, n)
+    ref = oracle.qft(x, n, phases)
+    c = ops.Circuit(n).qft(phases)
+    s = dev(x)
+    c.run(s, mode)
+    assert same_bits(host(s), ref)
+
+
+def test_qft_plus_state_known_answer():
+    # QFT of |+>^n is |0...0> (reference tests/gen/qft.hpp, c2-network.cpp:155-198)
+    n = 12
+    s = dev(np.full(1 << n, 2.0 ** (-n / 2), dtype=np.complex128))
+    ops.Circuit(n).qft().run(s)
+    out = host(s)
+    assert abs(out[0] - 1) < 1e-12
+    assert np.abs(out[1:]).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,depth", [(5, 6), (13, 3), (17, 2), (20, 2)])
+def test_random_circuit_bitexact_both_modes(n, depth):
+    c = ops.Circuit(n).random(depth, 100 + n)
+    x = rnd_state(n, 7)
+    ref = oracle_run(c, x)
+    for mode in (_lib.RUN_FUSED, _lib.RUN_PER_GATE):
+        s = dev(x)
+        c.run(s, mode)
+        assert same_bits(host(s), ref), mode
+
+
+@pytest.mark.parametrize("tb", [3, 5, 8, 10, 11, 12, 13])
+def test_every_tile_size_bitexact(tb):
+    n = 15
+    c = ops.Circuit(n).random(2, 5)
+    # also 3-qubit dense gates and 3-qubit diagonals
+    rng = np.random.default_rng(tb)
+    g3 = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    c.gate([14, 0, 7], g3).diag([3, 12, 1], np.exp(1j * rng.standard_normal(8))).gate([2, 9], rng.standard_normal(16))
+    x = rnd_state(n, 9)
+    ref = oracle_run(c, x)
+    ops.tune_circuit(tb)
+    try:
+        s = dev(x)
+        c.run(s)
+        assert same_bits(host(s), ref)
+    finally:
+        ops.tune_circuit(12)
+
+
+def test_batch_equals_individual_runs():
+    sizes = [3, 9, 12, 13, 14, 16, 12, 5]
+    circuits = [ops.Circuit(n).random(3, 40 + i) for i, n in enumerate(sizes)]
+    xs = [rnd_state(n, 60 + i) for i, n in enumerate(sizes)]
+    states = [dev(x) for x in xs]
+    ops.run_batch(circuits, states)
+    for c, x, s in zip(circuits, xs, states):
+        assert same_bits(host(s), oracle_run(c, x))
+
+
+def test_batch_rejects_aliased_states():
+    c = ops.Circuit(4).random(1, 1)
+    s = dev(rnd_state(4, 1))
+    with pytest.raises(_lib.QtnhError):
+        ops.run_batch([c, c], [s, s])
+
+
+@pytest.mark.parametrize("n", [1, 4, 11, 12, 13, 18])
+def test_expect_z_matches_brute_force(n):
+    x = rnd_state(n, 3 * n)
+    qs = list(range(n))
+    got = ops.expect_z(dev(x), n, qs)
+    p = np.abs(x) ** 2
+    idx = np.arange(1 << n)
+    for q, v in zip(qs, got):
+        bit = (idx >> (n - 1 - q)) & 1
+        assert abs(v - np.sum(p * (1 - 2 * bit))) < 1e-12
+
+
+def test_expect_z_basis_states_exact():
+    n = 14
+    for x0 in (0, 1, 5000, (1 << n) - 1):
+        e = np.zeros(1 << n, dtype=np.complex128)
+        e[x0] = 1
+        got = ops.expect_z(dev(e), n, list(range(n)))
+        want = [1.0 - 2.0 * ((x0 >> (n - 1 - q)) & 1) for q in range(n)]
+        assert got == want
+
+
+def test_large_random_circuit_keeps_norm_and_modes_agree():
+    n = 24
+    c = ops.Circuit(n).random(2, 77)
+    a = ops.fill_(ops.empty(1 << n), 5)
+    ops.scale_(a, 1 / np.sqrt(ops.norm2(a)))
+    b = a.clone()
+    c.run(a, _lib.RUN_FUSED)
+    c.run(b, _lib.RUN_PER_GATE)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.float64), b.view(torch.float64))
+    assert abs(ops.norm2(a) - 1) < 1e-12
+    z = ops.expect_z(a, n, [0, n // 2, n - 1])
+    assert all(abs(v) <= 1 for v in z)
