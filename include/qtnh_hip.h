@@ -156,6 +156,11 @@ qtnh_status qtnh_circuit_qft(qtnh_circuit_t c, const double *phases);
 /* Appends `depth` layers of a seeded random circuit: a Haar-random SU(2) on
  * every qubit, then CZ on the pairs of a seeded random pairing of the qubits. */
 qtnh_status qtnh_circuit_random(qtnh_circuit_t c, int depth, uint64_t seed);
+/* Appends `depth` layers on a rows x cols grid (qubit r*cols + c; rows*cols
+ * must equal the circuit's qubits): a Haar-random SU(2) on every qubit, then
+ * CZ on grid neighbours, cycling horizontal-even, horizontal-odd,
+ * vertical-even, vertical-odd couplers layer by layer. */
+qtnh_status qtnh_circuit_grid(qtnh_circuit_t c, int rows, int cols, int depth, uint64_t seed);
 /* Number of qubits, ops, and runs under the current tile size (any may be NULL). */
 qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, uint64_t *n_runs);
 /* Reads op i back: kind (0 gate, 1 diagonal), k, qubits[k], coefficients. */
@@ -166,6 +171,43 @@ qtnh_status qtnh_circuit_run(qtnh_ctx_t ctx, qtnh_circuit_t c, void *state, int 
 /* Applies circuit i to states[i] (distinct buffers) for every i, fused, with
  * one launch per run index across the whole batch. */
 qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit_t *circuits, void *const *states);
+
+/* ---- tensor networks: the caller of the contraction path
+ * (TensorNetwork::contractBond / contractAll, reference src/tensor/network.cpp:60-170).
+ * Leaf axes carry int64 bond labels; a label shared by two axes is a wire.
+ * Contractions use the default placement (open axes of the first tensor,
+ * then of the second; contract.cpp:219-248). ---- */
+typedef struct qtnh_net_s *qtnh_net_t;
+qtnh_status qtnh_net_create(qtnh_net_t *out);
+qtnh_status qtnh_net_destroy(qtnh_net_t net);
+/* Adds a leaf tensor (host data, row-major complex); *id (may be NULL) gets its node id. */
+qtnh_status qtnh_net_add_tensor(qtnh_net_t net, int rank, const uint64_t *dims, const int64_t *bonds,
+                                const double *data, int *id);
+/* Builds the amplitude network <bits| C |0...0> of a circuit: |0> vectors,
+ * one tensor per op (axes: inputs, then outputs; diagonals as dense tensors),
+ * and <bits_q| vectors (bits NULL: all zeros).  The network must be empty. */
+qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_t *bits);
+/* Greedy pairwise order (seeded ties) with fixed arena offsets for every
+ * intermediate.  QTNH_EUNSUPPORTED when an intermediate would exceed max_rank. */
+qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank);
+/* Plan summary: leaves, steps, largest rank, sum of 8 M N K, sum of
+ * 16 (|A| + |B| + |C|) bytes, device bytes of one run (any may be NULL). */
+qtnh_status qtnh_net_info(qtnh_net_t net, uint64_t *n_tensors, uint64_t *n_steps, int *max_rank, double *flops,
+                          double *bytes, uint64_t *device_bytes);
+qtnh_status qtnh_net_result_dims(qtnh_net_t net, int *rank, uint64_t *dims);
+/* Step s of the plan: node ids of its operands and result, and whether the
+ * small-contraction kernel (reference summation order) computes it. */
+qtnh_status qtnh_net_step(qtnh_net_t net, uint64_t s, int *a, int *b, int *c, int *direct);
+/* Contracts the planned network, one launch per step; out (host) receives the
+ * result (out_cap complex elements at least).  Blocks. */
+qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t out_cap);
+/* Contracts planned networks together: step k of every network in one grouped
+ * launch (small binary contractions) plus the large steps in turn.  Networks
+ * are admitted in consecutive waves whose device blocks fit budget_bytes.
+ * outs[i] (host) receives network i's result; results are bit-identical to
+ * qtnh_net_run.  *waves (may be NULL) gets the number of waves.  Blocks. */
+qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, double *const *outs,
+                               uint64_t budget_bytes, int *waves);
 
 /* ---- queries on a state (no reference counterpart) ---- */
 /* out[j] = <psi| Z_{qubits[j]} |psi> = sum_x |psi_x|^2 (1 - 2 x_q), fixed

@@ -61,6 +61,17 @@ SIGNATURES = {
     "qtnh_circuit_run": [_p, _p, _p, _i],
     "qtnh_circuit_run_batch": [_p, _i, ctypes.POINTER(_p), ctypes.POINTER(_p)],
     "qtnh_expect_z_c128": [_p, _p, _i, _i, _u16p, _dp],
+    "qtnh_circuit_grid": [_p, _i, _i, _i, _u64],
+    "qtnh_net_create": [ctypes.POINTER(_p)],
+    "qtnh_net_destroy": [_p],
+    "qtnh_net_add_tensor": [_p, _i, _u64p, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(_i)],
+    "qtnh_net_from_circuit": [_p, _p, ctypes.POINTER(ctypes.c_uint8)],
+    "qtnh_net_plan": [_p, _u64, _i],
+    "qtnh_net_info": [_p, _u64p, _u64p, ctypes.POINTER(_i), _dp, _dp, _u64p],
+    "qtnh_net_result_dims": [_p, ctypes.POINTER(_i), _u64p],
+    "qtnh_net_step": [_p, _u64, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "qtnh_net_run": [_p, _p, _dp, _u64],
+    "qtnh_net_run_batch": [_p, _i, ctypes.POINTER(_p), ctypes.POINTER(_dp), _u64, ctypes.POINTER(_i)],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_zgemm": None,

@@ -660,6 +660,40 @@ qtnh_status qtnh_circuit_random(qtnh_circuit_t c, int depth, uint64_t seed) {
   return QTNH_OK;
 }
 
+qtnh_status qtnh_circuit_grid(qtnh_circuit_t c, int rows, int cols, int depth, uint64_t seed) {
+  if (!c || rows < 1 || cols < 1 || depth < 0 || rows * cols != c->n) return QTNH_EINVAL;
+  HostRng rng(seed);
+  const double cz[8] = {1, 0, 1, 0, 1, 0, -1, 0};
+  for (int layer = 0; layer < depth; ++layer) {
+    for (int q = 0; q < c->n; ++q) {
+      double G[8];
+      random_su2(rng, G);
+      const uint16_t qq = (uint16_t)q;
+      qtnh_status st = qtnh_circuit_gate(c, 1, &qq, G);
+      if (st) return st;
+    }
+    // Couplers cycle through four patterns: horizontal even / odd columns,
+    // vertical even / odd rows.
+    const int pat = layer % 4;
+    for (int r = 0; r < rows; ++r)
+      for (int col = 0; col < cols; ++col) {
+        int r2 = r, c2 = col;
+        if (pat < 2) {
+          if ((col & 1) != pat) continue;
+          c2 = col + 1;
+        } else {
+          if ((r & 1) != pat - 2) continue;
+          r2 = r + 1;
+        }
+        if (r2 >= rows || c2 >= cols) continue;
+        const uint16_t q2[2] = {(uint16_t)(r * cols + col), (uint16_t)(r2 * cols + c2)};
+        qtnh_status st = qtnh_circuit_diag(c, 2, q2, cz);
+        if (st) return st;
+      }
+  }
+  return QTNH_OK;
+}
+
 qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, uint64_t *n_runs) {
   if (!c) return QTNH_EINVAL;
   if (n_qubits) *n_qubits = c->n;

@@ -246,9 +246,27 @@ static bool use_direct(const Plan &P) {
   return P.M * P.N * P.K <= (1ull << 18);
 }
 
+ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                             const uint16_t *r1, const uint16_t *r2, int precision) {
+  Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
+  ContractRoute R;
+  R.M = P.M;
+  R.N = P.N;
+  R.K = P.K;
+  R.direct = use_direct(P);
+  R.ws_bytes = 0;
+  if (!R.direct) {
+    // Same layout as launch_contract below: permuted A, permuted B, natural C, split scratch.
+    auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
+    R.ws_bytes = pad(P.M * P.K) + pad(P.K * P.N) + pad(P.nC);
+    if (precision == QTNH_PREC_SPLIT) R.ws_bytes += (split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull;
+  }
+  return R;
+}
+
 qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
                             const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *r1, const uint16_t *r2,
-                            c128 *C, int precision, double *err_out) {
+                            c128 *C, int precision, double *err_out, void *ext_ws, uint64_t ext_ws_bytes) {
   Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
   if (use_direct(P)) {
     // Small problems run exactly in FP64 whatever the precision asked for.
@@ -283,7 +301,12 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
   const uint64_t sbytes = split ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull) : 0;
   const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC)) + sbytes;
   char *ws = nullptr;
-  if (bytes) {
+  if (bytes && ext_ws) {
+    // A caller-provided region (batches, captured graphs): never reallocated here.
+    if (ext_ws_bytes < bytes) return set_error(ctx, QTNH_EINVAL, "contract: workspace of %llu bytes, need %llu",
+                                               (unsigned long long)ext_ws_bytes, (unsigned long long)bytes);
+    ws = (char *)ext_ws;
+  } else if (bytes) {
     void *w;
     qtnh_status st = workspace(ctx, bytes, &w);
     if (st) return st;

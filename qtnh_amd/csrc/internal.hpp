@@ -77,9 +77,18 @@ qtnh_status launch_gate(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *d
                         const uint16_t *axes, const double *G);
 qtnh_status launch_diag(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k,
                         const uint16_t *axes, const double *d);
+// ext_ws (optional): a workspace region used instead of the context's growable one.
 qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
                             const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
-                            const uint16_t *repl2, c128 *C, int precision, double *err_out);
+                            const uint16_t *repl2, c128 *C, int precision, double *err_out, void *ext_ws = nullptr,
+                            uint64_t ext_ws_bytes = 0);
+// Route launch_contract takes for these arguments, and its workspace need.
+struct ContractRoute {
+  bool direct;
+  uint64_t M, N, K, ws_bytes;
+};
+ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                             const uint16_t *repl1, const uint16_t *repl2, int precision);
 // Direct kernels only (reference summation order, no workspace use).
 qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
                                    const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,

@@ -1,0 +1,697 @@
+// Tensor networks contracted pairwise on the device: a seeded greedy order
+// planner, an allocation-free executor, and a batched executor that runs
+// step k of many networks together.
+//
+// The reference's caller is TensorNetwork::contractBond / contractAll
+// (src/tensor/network.cpp:60-170): each step is one Tensor::contract of two
+// tensors over the wires between them, default placement (open axes of the
+// first tensor, then of the second; src/tensor/contract.cpp:219-248).  Here a
+// network is a set of leaf tensors whose axes carry bond labels; a bond shared
+// by two tensors is a wire.
+//
+// Planning (host): the greedy order repeatedly contracts the pair of tensors
+// sharing a bond that minimises size(result) - size(a) - size(b), ties broken
+// by a seeded hash; intermediate tensors get fixed offsets in an arena by a
+// first-fit simulation of their lifetimes, so execution never allocates.
+//
+// Execution: one launch_contract per step (the one-at-a-time path), or, for a
+// batch, one grouped launch per step index covering every small contraction
+// of that index across the batch (binary dims), plus the large ones
+// individually.  Grouped and single launches compute the same sums in the
+// same order, so both executors give identical bits.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <queue>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace qtnh_hip {
+namespace {
+
+typedef double double2_t __attribute__((ext_vector_type(2)));
+
+constexpr int kGThreads = 256;
+constexpr int kGMaxBits = 32;
+
+// A small contraction over binary (dim 2) axes, as bit maps.
+struct GDesc {
+  const double2_t *A;
+  const double2_t *B;
+  double2_t *C;
+  uint64_t first;  // first block of this contraction in the launch
+  uint64_t nC;
+  int ncb;         // result bits
+  int kbits;       // closed bits
+  int8_t abit[kGMaxBits];  // A bit feeding result bit p (-1: from B)
+  int8_t bbit[kGMaxBits];
+  int8_t ka[kGMaxBits];    // A bit of closed bit j (j = 0: last wire in the sum order)
+  int8_t kb[kGMaxBits];
+};
+
+__global__ void __launch_bounds__(kGThreads) grouped_direct_kernel(const GDesc *__restrict__ descs, int nd,
+                                                                   uint64_t total_blocks) {
+  for (uint64_t blk = blockIdx.x; blk < total_blocks; blk += gridDim.x) {
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].first <= blk)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const GDesc &D = descs[lo];
+    const uint64_t e = (blk - D.first) * kGThreads + threadIdx.x;
+    if (e >= D.nC) continue;
+    uint64_t oa = 0, ob = 0;
+    for (int p = 0; p < D.ncb; ++p) {
+      if (!((e >> p) & 1ull)) continue;
+      if (D.abit[p] >= 0)
+        oa |= 1ull << D.abit[p];
+      else
+        ob |= 1ull << D.bbit[p];
+    }
+    // Closed tuples in the reference order: the tuple index counts up with the
+    // last wire fastest (bit 0).
+    double ar = 0.0, ai = 0.0;
+    const uint64_t K = 1ull << D.kbits;
+    for (uint64_t t = 0; t < K; ++t) {
+      uint64_t xa = oa, xb = ob;
+      for (int j = 0; j < D.kbits; ++j)
+        if ((t >> j) & 1ull) {
+          xa |= 1ull << D.ka[j];
+          xb |= 1ull << D.kb[j];
+        }
+      const double2_t a = D.A[xa], b = D.B[xb];
+      double pr, pi;
+      cmul_parts(a.x, a.y, b.x, b.y, &pr, &pi);
+      ar = ar + pr;
+      ai = ai + pi;
+    }
+    D.C[e] = double2_t{ar, ai};
+  }
+}
+
+// Copies results (src offset, count) to a packed output region.
+struct Gather {
+  const double2_t *src;
+  uint64_t n, dst;
+};
+
+__global__ void gather_kernel(const Gather *__restrict__ g, int ng, double2_t *__restrict__ out) {
+  const Gather G = g[blockIdx.x];
+  for (uint64_t i = threadIdx.x; i < G.n; i += blockDim.x) out[G.dst + i] = G.src[i];
+}
+
+uint64_t round16(uint64_t n) { return (n + 15) & ~15ull; }  // 256-byte granules (elements)
+
+uint64_t mix(uint64_t a, uint64_t b) {
+  uint64_t z = a + (b + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+}  // namespace qtnh_hip
+
+using qtnh_hip::c128;
+using qtnh_hip::GDesc;
+
+struct qtnh_net_s {
+  struct Node {
+    std::vector<uint64_t> dims;
+    std::vector<int64_t> bonds;
+    uint64_t size = 1;
+    uint64_t offset = 0;  // leaves: in the leaf block; intermediates: in the arena
+  };
+  struct Step {
+    int a, b, c;                  // node ids (a is the contraction's first tensor)
+    std::vector<uint16_t> wires;  // (axis of a, axis of b) pairs
+    bool direct = true;
+    bool binary = true;
+    uint64_t ws = 0;
+    double flops = 0, bytes = 0;
+  };
+  size_t n_leaves = 0;
+  std::vector<Node> nodes;          // leaves first, then one node per step
+  std::vector<double> leaf_data;    // interleaved complex, at the leaves' offsets
+  uint64_t leaf_elems = 0;
+  bool planned = false;
+  std::vector<Step> steps;
+  int result = -1;
+  uint64_t arena_elems = 0, ws_bytes = 0;
+  int max_rank = 0;
+  double flops = 0, bytes = 0;
+};
+
+namespace qtnh_hip {
+namespace {
+
+uint64_t block_elems(const qtnh_net_s &n) { return round16(n.leaf_elems) + round16(n.arena_elems); }
+
+// First-fit arena over element offsets; returns the high-water mark.
+struct Arena {
+  std::map<uint64_t, uint64_t> free_;  // start -> size
+  uint64_t top = 0;
+  uint64_t alloc(uint64_t n) {
+    n = round16(std::max<uint64_t>(n, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        const uint64_t s = it->first, sz = it->second;
+        free_.erase(it);
+        if (sz > n) free_[s + n] = sz - n;
+        return s;
+      }
+    }
+    const uint64_t s = top;
+    top += n;
+    return s;
+  }
+  void release(uint64_t s, uint64_t n) {
+    n = round16(std::max<uint64_t>(n, 1));
+    auto it = free_.emplace(s, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+        it = pv;
+      }
+    }
+    if (it->first + it->second == top) {
+      top = it->first;
+      free_.erase(it);
+    }
+  }
+};
+
+qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
+  net.steps.clear();
+  net.nodes.resize(net.n_leaves);
+  net.planned = false;
+  net.max_rank = 0;
+  net.flops = net.bytes = 0;
+  const size_t L = net.n_leaves;
+  if (L == 0) return QTNH_EINVAL;
+  std::vector<char> live(L, 1);
+  std::map<int64_t, std::vector<int>> holders;  // bond -> live nodes carrying it
+  for (size_t i = 0; i < L; ++i) {
+    net.max_rank = std::max(net.max_rank, (int)net.nodes[i].dims.size());
+    for (int64_t b : net.nodes[i].bonds) holders[b].push_back((int)i);
+  }
+  for (auto &h : holders)
+    if (h.second.size() > 2) return QTNH_EINVAL;  // a bond joins at most two axes
+  // Result of contracting a and b: open axes of a, then open axes of b.
+  auto merged = [&](int a, int b, qtnh_net_s::Node &out, std::vector<uint16_t> &wires) {
+    const auto &A = net.nodes[a], &B = net.nodes[b];
+    out = qtnh_net_s::Node();
+    wires.clear();
+    std::vector<char> bused(B.bonds.size(), 0);
+    for (size_t i = 0; i < A.bonds.size(); ++i) {
+      int j = -1;
+      for (size_t k = 0; k < B.bonds.size(); ++k)
+        if (B.bonds[k] == A.bonds[i]) j = (int)k;
+      if (j >= 0) {
+        wires.push_back((uint16_t)i);
+        wires.push_back((uint16_t)j);
+        bused[j] = 1;
+      } else {
+        out.dims.push_back(A.dims[i]);
+        out.bonds.push_back(A.bonds[i]);
+      }
+    }
+    for (size_t k = 0; k < B.bonds.size(); ++k)
+      if (!bused[k]) {
+        out.dims.push_back(B.dims[k]);
+        out.bonds.push_back(B.bonds[k]);
+      }
+    out.size = 1;
+    for (uint64_t d : out.dims) out.size *= d;
+  };
+  struct Cand {
+    double cost;
+    uint64_t tie;
+    int a, b;
+    bool operator<(const Cand &o) const { return cost != o.cost ? cost > o.cost : tie > o.tie; }
+  };
+  std::priority_queue<Cand> pq;
+  auto push_pair = [&](int a, int b) {
+    if (a > b) std::swap(a, b);
+    qtnh_net_s::Node r;
+    std::vector<uint16_t> w;
+    merged(a, b, r, w);
+    const double cost = (double)r.size - (double)net.nodes[a].size - (double)net.nodes[b].size;
+    pq.push({cost, mix(seed, ((uint64_t)a << 32) | (uint64_t)b), a, b});
+  };
+  for (auto &h : holders)
+    if (h.second.size() == 2) push_pair(h.second[0], h.second[1]);
+  std::vector<uint64_t> offset(L, 0);
+  Arena arena;
+  size_t live_count = L;
+  while (live_count > 1) {
+    int a = -1, b = -1;
+    while (!pq.empty()) {
+      Cand c = pq.top();
+      pq.pop();
+      if (live[c.a] && live[c.b]) {
+        a = c.a;
+        b = c.b;
+        break;
+      }
+    }
+    if (a < 0) {
+      // Disconnected components: outer product of the two smallest live tensors.
+      std::vector<int> lv;
+      for (size_t i = 0; i < live.size(); ++i)
+        if (live[i]) lv.push_back((int)i);
+      std::sort(lv.begin(), lv.end(), [&](int x, int y) { return net.nodes[x].size < net.nodes[y].size; });
+      a = std::min(lv[0], lv[1]);
+      b = std::max(lv[0], lv[1]);
+    }
+    qtnh_net_s::Step st;
+    qtnh_net_s::Node r;
+    merged(a, b, r, st.wires);
+    if ((int)r.dims.size() > max_rank || (int)r.dims.size() > QTNH_MAX_RANK) return QTNH_EUNSUPPORTED;
+    st.a = a;
+    st.b = b;
+    st.c = (int)net.nodes.size();
+    const auto &A = net.nodes[a], &B = net.nodes[b];
+    uint64_t K = 1;
+    for (size_t w = 0; w < st.wires.size(); w += 2) K *= A.dims[st.wires[w]];
+    st.flops = 8.0 * (double)r.size * (double)K;
+    st.bytes = 16.0 * ((double)A.size + (double)B.size + (double)r.size);
+    for (uint64_t d : A.dims) st.binary = st.binary && d == 2;
+    for (uint64_t d : B.dims) st.binary = st.binary && d == 2;
+    st.binary = st.binary && A.dims.size() <= kGMaxBits && B.dims.size() <= kGMaxBits && r.dims.size() <= kGMaxBits;
+    // Route and workspace, as launch_contract will choose them.
+    std::vector<uint16_t> r1(A.dims.size()), r2(B.dims.size());
+    default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, (int)st.wires.size() / 2, st.wires.data(), r1.data(),
+                  r2.data());
+    ContractRoute R = contract_route((int)A.dims.size(), A.dims.data(), (int)B.dims.size(), B.dims.data(),
+                                     (int)st.wires.size() / 2, st.wires.data(), r1.data(), r2.data(), QTNH_PREC_FP64);
+    st.direct = R.direct;
+    st.ws = R.ws_bytes;
+    net.ws_bytes = std::max(net.ws_bytes, st.ws);
+    // Arena: the result is placed before its inputs are released (no aliasing).
+    r.offset = arena.alloc(r.size);
+    net.max_rank = std::max(net.max_rank, (int)r.dims.size());
+    net.flops += st.flops;
+    net.bytes += st.bytes;
+    net.nodes.push_back(r);
+    live.push_back(1);
+    for (int x : {a, b}) {
+      live[x] = 0;
+      if ((size_t)x >= L) arena.release(net.nodes[x].offset, net.nodes[x].size);
+    }
+    --live_count;
+    net.steps.push_back(st);
+    // New candidate pairs through the result's bonds.
+    const int c = st.c;
+    for (int64_t bond : net.nodes[c].bonds) {
+      auto &h = holders[bond];
+      for (int &x : h)
+        if (x == a || x == b) x = c;
+      if (h.size() == 2 && live[h[0]] && live[h[1]]) push_pair(h[0], h[1]);
+    }
+  }
+  // High-water mark of the arena.
+  uint64_t hw = 0;
+  for (const auto &st : net.steps) {
+    const auto &r = net.nodes[st.c];
+    hw = std::max(hw, r.offset + round16(std::max<uint64_t>(r.size, 1)));
+  }
+  net.arena_elems = hw;
+  net.result = net.steps.empty() ? 0 : net.steps.back().c;
+  net.planned = true;
+  return QTNH_OK;
+}
+
+// Device placement of one network: its leaves and its arena.
+struct Where {
+  c128 *leaves;
+  c128 *arena;
+};
+
+c128 *node_ptr(const qtnh_net_s &net, Where block, int x) {
+  if ((size_t)x < net.n_leaves) return block.leaves + net.nodes[x].offset;
+  return block.arena + net.nodes[x].offset;
+}
+
+qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t s, void *ws, uint64_t ws_bytes) {
+  const auto &st = net.steps[s];
+  const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
+  std::vector<uint16_t> r1(std::max<size_t>(A.dims.size(), 1)), r2(std::max<size_t>(B.dims.size(), 1));
+  const int nw = (int)st.wires.size() / 2;
+  default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
+  return launch_contract(ctx, node_ptr(net, block, st.a), (int)A.dims.size(), A.dims.data(), node_ptr(net, block, st.b),
+                         (int)B.dims.size(), B.dims.data(), nw, st.wires.data(), r1.data(), r2.data(),
+                         node_ptr(net, block, st.c), QTNH_PREC_FP64, nullptr, ws, ws_bytes);
+}
+
+GDesc make_gdesc(const qtnh_net_s &net, Where block, size_t s, uint64_t first) {
+  const auto &st = net.steps[s];
+  const auto &A = net.nodes[st.a], &B = net.nodes[st.b], &C = net.nodes[st.c];
+  const int rA = (int)A.dims.size(), rB = (int)B.dims.size(), rC = (int)C.dims.size();
+  GDesc d;
+  memset(&d, -1, sizeof(d));
+  d.A = (const double2_t *)node_ptr(net, block, st.a);
+  d.B = (const double2_t *)node_ptr(net, block, st.b);
+  d.C = (double2_t *)node_ptr(net, block, st.c);
+  d.first = first;
+  d.nC = C.size;
+  d.ncb = rC;
+  const int nw = (int)st.wires.size() / 2;
+  d.kbits = nw;
+  // Result axis q <- open axes of A in order, then of B; bit p = rC - 1 - q.
+  std::vector<char> wa(rA, 0), wb(rB, 0);
+  std::vector<std::pair<int, int>> w(nw);
+  for (int i = 0; i < nw; ++i) {
+    w[i] = {st.wires[2 * i], st.wires[2 * i + 1]};
+    wa[w[i].first] = 1;
+    wb[w[i].second] = 1;
+  }
+  int q = 0;
+  for (int i = 0; i < rA; ++i)
+    if (!wa[i]) {
+      d.abit[rC - 1 - q] = (int8_t)(rA - 1 - i);
+      ++q;
+    }
+  for (int i = 0; i < rB; ++i)
+    if (!wb[i]) {
+      d.abit[rC - 1 - q] = -1;
+      d.bbit[rC - 1 - q] = (int8_t)(rB - 1 - i);
+      ++q;
+    }
+  // Sum order: wires sorted by their axis in B, the last one fastest.
+  std::sort(w.begin(), w.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+    return x.second < y.second;
+  });
+  for (int j = 0; j < nw; ++j) {
+    d.ka[j] = (int8_t)(rA - 1 - w[nw - 1 - j].first);
+    d.kb[j] = (int8_t)(rB - 1 - w[nw - 1 - j].second);
+  }
+  return d;
+}
+
+uint64_t gdesc_blocks(const GDesc &d) { return (d.nC + kGThreads - 1) / kGThreads; }
+
+}  // namespace
+}  // namespace qtnh_hip
+
+using namespace qtnh_hip;
+
+extern "C" {
+
+qtnh_status qtnh_net_create(qtnh_net_t *out) {
+  if (!out) return QTNH_EINVAL;
+  *out = new (std::nothrow) qtnh_net_s;
+  return *out ? QTNH_OK : QTNH_ENOMEM;
+}
+
+qtnh_status qtnh_net_destroy(qtnh_net_t net) {
+  if (!net) return QTNH_EINVAL;
+  delete net;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_add_tensor(qtnh_net_t net, int rank, const uint64_t *dims, const int64_t *bonds,
+                                const double *data, int *id) {
+  if (!net || rank < 0 || rank > QTNH_MAX_RANK || (rank > 0 && (!dims || !bonds)) || !data) return QTNH_EINVAL;
+  if (net->nodes.size() != net->n_leaves) return QTNH_EINVAL;  // leaves only before planning
+  qtnh_net_s::Node x;
+  uint64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] == 0 || n > UINT64_MAX / dims[i]) return QTNH_EINVAL;
+    for (int j = 0; j < i; ++j)
+      if (bonds[j] == bonds[i]) return QTNH_EINVAL;
+    n *= dims[i];
+    x.dims.push_back(dims[i]);
+    x.bonds.push_back(bonds[i]);
+  }
+  x.size = n;
+  x.offset = round16(net->leaf_elems);
+  net->leaf_elems = x.offset + n;
+  net->leaf_data.resize(2 * net->leaf_elems, 0.0);
+  memcpy(net->leaf_data.data() + 2 * x.offset, data, n * sizeof(c128));
+  if (id) *id = (int)net->nodes.size();
+  net->nodes.push_back(x);
+  net->n_leaves = net->nodes.size();
+  net->planned = false;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_t *bits) {
+  if (!net || !c) return QTNH_EINVAL;
+  if (!net->nodes.empty()) return QTNH_EINVAL;
+  int n = 0;
+  uint64_t nops = 0;
+  qtnh_status st = qtnh_circuit_info(c, &n, &nops, nullptr);
+  if (st) return st;
+  std::vector<int64_t> wire(n);
+  int64_t next = 0;
+  const uint64_t d1[1] = {2};
+  for (int q = 0; q < n; ++q) {
+    wire[q] = next++;
+    const double ket0[4] = {1, 0, 0, 0};
+    st = qtnh_net_add_tensor(net, 1, d1, &wire[q], ket0, nullptr);
+    if (st) return st;
+  }
+  for (uint64_t i = 0; i < nops; ++i) {
+    int kind = 0, k = 0;
+    uint16_t qs[3];
+    double co[128];
+    st = qtnh_circuit_op(c, i, &kind, &k, qs, co);
+    if (st) return st;
+    const int K = 1 << k;
+    std::vector<double> G(2 * K * K, 0.0);
+    if (kind == 0) {
+      memcpy(G.data(), co, sizeof(double) * 2 * K * K);
+    } else {
+      for (int x = 0; x < K; ++x) {
+        G[2 * (x * K + x)] = co[2 * x];
+        G[2 * (x * K + x) + 1] = co[2 * x + 1];
+      }
+    }
+    // Tensor axes: inputs (current wires of the op's qubits), then outputs.
+    std::vector<uint64_t> dims(2 * k, 2);
+    std::vector<int64_t> bonds(2 * k);
+    for (int j = 0; j < k; ++j) {
+      bonds[j] = wire[qs[j]];
+      wire[qs[j]] = next++;
+      bonds[k + j] = wire[qs[j]];
+    }
+    st = qtnh_net_add_tensor(net, 2 * k, dims.data(), bonds.data(), G.data(), nullptr);
+    if (st) return st;
+  }
+  for (int q = 0; q < n; ++q) {
+    const int b = bits ? (bits[q] & 1) : 0;
+    const double bra[4] = {b ? 0.0 : 1.0, 0, b ? 1.0 : 0.0, 0};
+    st = qtnh_net_add_tensor(net, 1, d1, &wire[q], bra, nullptr);
+    if (st) return st;
+  }
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank) {
+  if (!net || max_rank < 0) return QTNH_EINVAL;
+  return plan_greedy(*net, seed, max_rank);
+}
+
+qtnh_status qtnh_net_info(qtnh_net_t net, uint64_t *n_tensors, uint64_t *n_steps, int *max_rank, double *flops,
+                          double *bytes, uint64_t *device_bytes) {
+  if (!net) return QTNH_EINVAL;
+  if (n_tensors) *n_tensors = net->n_leaves;
+  if (n_steps) *n_steps = net->steps.size();
+  if (max_rank) *max_rank = net->max_rank;
+  if (flops) *flops = net->flops;
+  if (bytes) *bytes = net->bytes;
+  if (device_bytes) *device_bytes = block_elems(*net) * sizeof(c128) + net->ws_bytes;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_result_dims(qtnh_net_t net, int *rank, uint64_t *dims) {
+  if (!net || !rank) return QTNH_EINVAL;
+  if (!net->planned) return QTNH_EINVAL;
+  const auto &r = net->nodes[net->result];
+  *rank = (int)r.dims.size();
+  if (dims)
+    for (size_t i = 0; i < r.dims.size(); ++i) dims[i] = r.dims[i];
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_step(qtnh_net_t net, uint64_t s, int *a, int *b, int *c, int *direct) {
+  if (!net || s >= net->steps.size()) return QTNH_EINVAL;
+  const auto &st = net->steps[s];
+  if (a) *a = st.a;
+  if (b) *b = st.b;
+  if (c) *c = st.c;
+  if (direct) *direct = st.direct;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t out_cap) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!net || !out) return set_error(ctx, QTNH_EINVAL, "net_run: null argument");
+  if (!net->planned) return set_error(ctx, QTNH_EINVAL, "net_run: network has no plan");
+  const uint64_t rsize = net->nodes[net->result].size;
+  if (out_cap < rsize) return set_error(ctx, QTNH_EINVAL, "net_run: output holds %llu, result has %llu elements",
+                                        (unsigned long long)out_cap, (unsigned long long)rsize);
+  const uint64_t elems = block_elems(*net);
+  void *w = nullptr;
+  st = workspace(ctx, elems * sizeof(c128) + net->ws_bytes, &w);
+  if (st) return st;
+  Where block = {(c128 *)w, (c128 *)w + round16(net->leaf_elems)};
+  char *ws = (char *)((c128 *)w + elems);
+  st = check_hip(ctx, hipMemcpyAsync(block.leaves, net->leaf_data.data(), net->leaf_elems * sizeof(c128),
+                                     hipMemcpyHostToDevice, ctx->stream),
+                 "net_run leaf upload");
+  if (st) return st;
+  for (size_t s = 0; s < net->steps.size(); ++s) {
+    st = run_step(ctx, *net, block, s, ws, net->ws_bytes);
+    if (st) return st;
+  }
+  st = check_hip(ctx, hipMemcpyAsync(out, node_ptr(*net, block, net->result), rsize * sizeof(c128),
+                                     hipMemcpyDeviceToHost, ctx->stream),
+                 "net_run result copy");
+  if (st) return st;
+  return check_hip(ctx, hipStreamSynchronize(ctx->stream), "net_run sync");
+}
+
+qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, double *const *outs,
+                               uint64_t budget_bytes, int *waves_out) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (count < 0 || (count > 0 && (!nets || !outs))) return set_error(ctx, QTNH_EINVAL, "net_run_batch: bad arguments");
+  for (int i = 0; i < count; ++i)
+    if (!nets[i] || !outs[i] || !nets[i]->planned)
+      return set_error(ctx, QTNH_EINVAL, "net_run_batch: network %d is null or has no plan", i);
+  // Admission: waves of consecutive networks whose blocks fit the budget
+  // (the GEMM workspace is shared by the wave, whose large steps run in turn).
+  int waves = 0;
+  for (int w0 = 0; w0 < count;) {
+    uint64_t elems = 0, wsb = 0;
+    int w1 = w0;
+    while (w1 < count) {
+      const uint64_t e2 = elems + block_elems(*nets[w1]);
+      const uint64_t ws2 = std::max(wsb, nets[w1]->ws_bytes);
+      if (w1 > w0 && e2 * sizeof(c128) + ws2 > budget_bytes) break;
+      elems = e2;
+      wsb = ws2;
+      ++w1;
+    }
+    ++waves;
+    // Layout: [leaves of every net | arenas | results | gemm workspace]; the
+    // leaves go up in one copy, the arenas are written before they are read.
+    uint64_t res_elems = 0, leaf_elems = 0;
+    for (int i = w0; i < w1; ++i) {
+      res_elems += nets[i]->nodes[nets[i]->result].size;
+      leaf_elems += round16(nets[i]->leaf_elems);
+    }
+    const uint64_t total = (elems + round16(res_elems)) * sizeof(c128) + wsb;
+    void *wp = nullptr;
+    st = workspace(ctx, total, &wp);
+    if (st) return st;
+    c128 *base = (c128 *)wp;
+    c128 *res = base + elems;
+    char *gws = (char *)(res + round16(res_elems));
+    std::vector<Where> blocks(w1 - w0);
+    std::vector<double> leaves(2 * std::max<uint64_t>(leaf_elems, 1), 0.0);
+    uint64_t loff = 0, aoff = leaf_elems;
+    size_t steps = 0;
+    for (int i = w0; i < w1; ++i) {
+      blocks[i - w0] = {base + loff, base + aoff};
+      memcpy(leaves.data() + 2 * loff, nets[i]->leaf_data.data(), nets[i]->leaf_elems * sizeof(c128));
+      loff += round16(nets[i]->leaf_elems);
+      aoff += round16(nets[i]->arena_elems);
+      steps = std::max(steps, nets[i]->steps.size());
+    }
+    st = check_hip(ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice,
+                                       ctx->stream),
+                   "batch leaf upload");
+    if (st) return st;
+    // Grouped descriptors for every step index, uploaded once.
+    std::vector<GDesc> descs;
+    std::vector<size_t> starts(steps + 1, 0);
+    for (size_t s = 0; s < steps; ++s) {
+      starts[s] = descs.size();
+      uint64_t first = 0;
+      for (int i = w0; i < w1; ++i) {
+        const auto &n = *nets[i];
+        if (s >= n.steps.size() || !n.steps[s].direct || !n.steps[s].binary) continue;
+        GDesc d = make_gdesc(n, blocks[i - w0], s, first);
+        first += gdesc_blocks(d);
+        descs.push_back(d);
+      }
+    }
+    starts[steps] = descs.size();
+    std::vector<Gather> gathers;
+    uint64_t dst = 0;
+    for (int i = w0; i < w1; ++i) {
+      const auto &n = *nets[i];
+      gathers.push_back({(const double2_t *)node_ptr(n, blocks[i - w0], n.result), n.nodes[n.result].size, dst});
+      dst += n.nodes[n.result].size;
+    }
+    const uint64_t dbytes = descs.size() * sizeof(GDesc) + gathers.size() * sizeof(Gather) + 256;
+    st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, dbytes);
+    if (st) return st;
+    GDesc *ddesc = (GDesc *)ctx->desc;
+    Gather *dgath = (Gather *)((char *)ctx->desc + ((descs.size() * sizeof(GDesc) + 255) & ~255ull));
+    if (!descs.empty()) {
+      st = check_hip(ctx, hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(GDesc), hipMemcpyHostToDevice,
+                                         ctx->stream),
+                     "batch descriptor upload");
+      if (st) return st;
+    }
+    st = check_hip(ctx, hipMemcpyAsync(dgath, gathers.data(), gathers.size() * sizeof(Gather), hipMemcpyHostToDevice,
+                                       ctx->stream),
+                   "batch gather upload");
+    if (st) return st;
+    for (size_t s = 0; s < steps; ++s) {
+      const size_t b = starts[s], e = starts[s + 1];
+      if (e > b) {
+        const uint64_t tb = descs[e - 1].first + gdesc_blocks(descs[e - 1]);
+        const unsigned grid = (unsigned)std::min<uint64_t>(tb, (uint64_t)ctx->num_cus * 16);
+        hipLaunchKernelGGL(grouped_direct_kernel, dim3(grid), dim3(kGThreads), 0, ctx->stream, ddesc + b, (int)(e - b),
+                           tb);
+        st = check_hip(ctx, hipGetLastError(), "grouped contraction launch");
+        if (st) return st;
+      }
+      for (int i = w0; i < w1; ++i) {
+        const auto &n = *nets[i];
+        if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary)) continue;
+        st = run_step(ctx, n, blocks[i - w0], s, gws, wsb);
+        if (st) return st;
+      }
+    }
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)gathers.size()), dim3(64), 0, ctx->stream, dgath,
+                       (int)gathers.size(), (double2_t *)res);
+    st = check_hip(ctx, hipGetLastError(), "gather launch");
+    if (st) return st;
+    std::vector<double> hres(2 * std::max<uint64_t>(res_elems, 1));
+    st = check_hip(ctx, hipMemcpyAsync(hres.data(), res, res_elems * sizeof(c128), hipMemcpyDeviceToHost, ctx->stream),
+                   "batch result copy");
+    if (st) return st;
+    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+    if (st) return st;
+    uint64_t r = 0;
+    for (int i = w0; i < w1; ++i) {
+      const uint64_t n = nets[i]->nodes[nets[i]->result].size;
+      memcpy(outs[i], hres.data() + 2 * r, n * sizeof(c128));
+      r += n;
+    }
+    w0 = w1;
+  }
+  if (waves_out) *waves_out = waves;
+  return QTNH_OK;
+}
+
+}  // extern "C"

@@ -288,6 +288,10 @@ class Circuit:
         check(lib().qtnh_circuit_random(self._h, depth, seed))
         return self
 
+    def grid(self, rows, cols, depth, seed):
+        check(lib().qtnh_circuit_grid(self._h, rows, cols, depth, seed))
+        return self
+
     def info(self):
         n = ctypes.c_int(0)
         nops = ctypes.c_uint64(0)
@@ -347,3 +351,95 @@ def expect_z(state, n_qubits, qubits):
     check(lib().qtnh_expect_z_c128(ctx, _ptr(state), n_qubits, len(qubits), qp,
                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), ctx)
     return [float(v) for v in out[:len(qubits)]]
+
+
+class Network:
+    """A tensor network of leaf tensors with labelled bonds, planned and
+    contracted pairwise on the device (reference TensorNetwork::contractAll)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().qtnh_net_create(ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().qtnh_net_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def add(self, dims, bonds, data):
+        d, dp = _u64(dims)
+        b = np.ascontiguousarray(np.asarray(list(bonds) or [0], dtype=np.int64))
+        x, xp = _c128_host(data)
+        if x.size != _size(dims):
+            raise ValueError("tensor data has %d elements, dims need %d" % (x.size, _size(dims)))
+        i = ctypes.c_int(0)
+        check(lib().qtnh_net_add_tensor(self._h, len(dims), dp, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        xp, ctypes.byref(i)))
+        return i.value
+
+    @classmethod
+    def amplitude(cls, circuit, bits=None):
+        """The network of <bits| circuit |0...0>."""
+        net = cls()
+        if bits is None:
+            check(lib().qtnh_net_from_circuit(net._h, circuit.handle, None))
+        else:
+            bb = np.ascontiguousarray(np.asarray(bits, dtype=np.uint8))
+            if bb.size != circuit.n:
+                raise ValueError("one bit per qubit")
+            check(lib().qtnh_net_from_circuit(net._h, circuit.handle, bb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return net
+
+    def plan(self, seed=0, max_rank=26):
+        check(lib().qtnh_net_plan(self._h, seed, max_rank))
+        return self
+
+    def info(self):
+        nt = ctypes.c_uint64(0)
+        ns = ctypes.c_uint64(0)
+        mr = ctypes.c_int(0)
+        fl = ctypes.c_double(0)
+        by = ctypes.c_double(0)
+        db = ctypes.c_uint64(0)
+        check(lib().qtnh_net_info(self._h, ctypes.byref(nt), ctypes.byref(ns), ctypes.byref(mr), ctypes.byref(fl),
+                                  ctypes.byref(by), ctypes.byref(db)))
+        return {"tensors": nt.value, "steps": ns.value, "max_rank": mr.value, "flops": fl.value,
+                "bytes": by.value, "device_bytes": db.value}
+
+    def result_dims(self):
+        r = ctypes.c_int(0)
+        d = np.zeros(64, dtype=np.uint64)
+        check(lib().qtnh_net_result_dims(self._h, ctypes.byref(r), d.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return [int(v) for v in d[:r.value]]
+
+    def run(self, device=None):
+        n = _size(self.result_dims())
+        out = np.zeros(max(n, 1), dtype=np.complex128)
+        ctx = context(device)
+        check(lib().qtnh_net_run(ctx, self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n), ctx)
+        return out[:n]
+
+
+def run_networks(nets, budget_bytes=1 << 40, device=None):
+    """Contract planned networks as one batch; returns (results, waves)."""
+    outs = [np.zeros(max(_size(n.result_dims()), 1), dtype=np.complex128) for n in nets]
+    if not nets:
+        return outs, 0
+    ctx = context(device)
+    hs = (ctypes.c_void_p * len(nets))(*[n.handle.value for n in nets])
+    ps = (ctypes.POINTER(ctypes.c_double) * len(nets))(*[o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                                                          for o in outs])
+    waves = ctypes.c_int(0)
+    check(lib().qtnh_net_run_batch(ctx, len(nets), hs, ps, budget_bytes, ctypes.byref(waves)), ctx)
+    return [o[:_size(n.result_dims())] for o, n in zip(outs, nets)], waves.value

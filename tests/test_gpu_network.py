@@ -1,0 +1,115 @@
+"""Networks contracted on the device, against independent answers.
+
+  * amplitude networks of grid circuits equal the state-vector amplitude of
+    the same circuit (fused circuit executor) within 1e-12 relative;
+  * two plans (different seeds, hence orders) agree within 1e-12 relative;
+  * general networks equal numpy einsum over the same bond labels;
+  * the batched executor (grouped launches, admission waves) equals the
+    one-at-a-time executor bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from qtnh_amd import ops
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def statevector_amplitude(c, bits):
+    n = c.n
+    s = torch.zeros(1 << n, dtype=torch.complex128, device="cuda")
+    s[0] = 1
+    c.run(s)
+    idx = sum(int(b) << (n - 1 - q) for q, b in enumerate(bits))
+    torch.cuda.synchronize()
+    return complex(s[idx].item()), s
+
+
+@pytest.mark.parametrize("rows,cols,depth", [(2, 2, 4), (3, 3, 6), (3, 4, 8), (4, 4, 8)])
+def test_amplitude_matches_state_vector(rows, cols, depth):
+    n = rows * cols
+    c = ops.Circuit(n).grid(rows, cols, depth, 1000 + n)
+    rng = np.random.default_rng(n)
+    bits = rng.integers(0, 2, n)
+    want, psi = statevector_amplitude(c, bits)
+    got = ops.Network.amplitude(c, bits).plan(3).run()
+    assert got.shape == (1,)
+    # amplitudes are O(2^-n/2): relative to the larger of |want| and that scale
+    assert abs(got[0] - want) <= 1e-12 * max(abs(want), 2.0 ** (-n / 2))
+
+
+def test_order_independence():
+    c = ops.Circuit(12).grid(3, 4, 10, 5)
+    bits = [1, 0] * 6
+    vals = [ops.Network.amplitude(c, bits).plan(seed).run()[0] for seed in (0, 1, 2, 99)]
+    for v in vals[1:]:
+        assert abs(v - vals[0]) <= 1e-12 * max(abs(vals[0]), 2.0 ** -6)
+
+
+def test_general_network_matches_einsum():
+    rng = np.random.default_rng(4)
+    dims = {0: 3, 1: 4, 2: 5, 3: 2, 4: 6, 5: 3}
+    tensors = [[0, 1], [1, 2, 3], [3, 4], [2, 4, 5], [0, 5]]
+    letters = "abcdef"
+    net = ops.Network()
+    arrays = []
+    for bonds in tensors:
+        shape = [dims[b] for b in bonds]
+        x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        arrays.append(x)
+        net.add(shape, bonds, x.reshape(-1))
+    got = net.plan(1).run()
+    want = np.einsum(",".join("".join(letters[b] for b in t) for t in tensors) + "->", *arrays)
+    assert abs(got[0] - want) <= 1e-12 * abs(want)
+
+
+def test_open_network_matches_einsum():
+    rng = np.random.default_rng(8)
+    net = ops.Network()
+    a = rng.standard_normal((2, 3, 4)) + 1j * rng.standard_normal((2, 3, 4))
+    b = rng.standard_normal((4, 5)) + 1j * rng.standard_normal((4, 5))
+    net.add([2, 3, 4], [10, 11, 12], a.reshape(-1))
+    net.add([4, 5], [12, 13], b.reshape(-1))
+    got = net.plan().run()
+    assert net.result_dims() == [2, 3, 5]
+    want = np.einsum("abc,cd->abd", a, b).reshape(-1)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def mixed_networks():
+    nets = []
+    for i, (r, c, d) in enumerate([(2, 2, 3), (3, 3, 8), (4, 4, 8), (3, 4, 12), (4, 5, 6), (2, 3, 5), (4, 4, 12)]):
+        circ = ops.Circuit(r * c).grid(r, c, d, 500 + i)
+        bits = np.random.default_rng(i).integers(0, 2, r * c)
+        nets.append(ops.Network.amplitude(circ, bits).plan(i))
+    return nets
+
+
+def test_batch_equals_one_at_a_time_bitwise():
+    nets = mixed_networks()
+    solo = [n.run() for n in nets]
+    batch, waves = ops.run_networks(nets)
+    assert waves == 1
+    for s, b in zip(solo, batch):
+        assert same_bits(s, b)
+
+
+def test_batch_waves_under_a_small_budget():
+    nets = mixed_networks()
+    solo = [n.run() for n in nets]
+    biggest = max(n.info()["device_bytes"] for n in nets)
+    batch, waves = ops.run_networks(nets, budget_bytes=biggest)
+    assert waves > 1
+    for s, b in zip(solo, batch):
+        assert same_bits(s, b)
