@@ -209,6 +209,34 @@ qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t o
 qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, double *const *outs,
                                uint64_t budget_bytes, int *waves);
 
+/* ---- serving loop (no reference counterpart): queries on the output states of
+ * seeded random SU(2) + CZ circuits (qtnh_circuit_random), answered on the
+ * wall clock under an HBM budget ---- */
+#define QTNH_QUERY_AMPLITUDE 0 /* value = <basis| C |0...0> */
+#define QTNH_QUERY_EXPECT_Z 1  /* value = (<Z_0>, <Z_{n-1}>), zsum = sum_q <Z_q> */
+#define QTNH_SERVE_SEQUENTIAL 0 /* one query at a time, one launch per gate, no resident states */
+#define QTNH_SERVE_BATCHED 1    /* FIFO admission under the budget, fused batched runs by size class,
+                                   least-recently-used resident output states */
+typedef struct {
+  int n_qubits;
+  int depth;
+  uint64_t seed;
+  int kind;
+  uint64_t basis;   /* amplitude index (qubit 0 most significant) */
+  double arrival_s; /* seconds after the start of qtnh_serve */
+} qtnh_query;
+typedef struct {
+  double start_s, done_s; /* seconds after the start */
+  double value[2];
+  double zsum;
+  int batch;  /* admission round that answered it */
+  int cached; /* 1: answered from a resident state */
+} qtnh_query_result;
+/* Serves the queries in arrival order; returns when all are answered.
+ * Answers are identical under both policies. */
+qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *queries, qtnh_query_result *results,
+                       uint64_t budget_bytes, int policy);
+
 /* ---- queries on a state (no reference counterpart) ---- */
 /* out[j] = <psi| Z_{qubits[j]} |psi> = sum_x |psi_x|^2 (1 - 2 x_q), fixed
  * reduction order; blocks until the values are on the host. */

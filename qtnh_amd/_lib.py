@@ -13,6 +13,18 @@ QTNH_OK, QTNH_EINVAL, QTNH_ENOMEM, QTNH_EHIP, QTNH_EUNSUPPORTED = range(5)
 X = 0xFFFF
 PREC_FP64, PREC_SPLIT = 0, 1
 RUN_PER_GATE, RUN_FUSED = 0, 1
+QUERY_AMPLITUDE, QUERY_EXPECT_Z = 0, 1
+SERVE_SEQUENTIAL, SERVE_BATCHED = 0, 1
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("n_qubits", ctypes.c_int), ("depth", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("kind", ctypes.c_int), ("basis", ctypes.c_uint64), ("arrival_s", ctypes.c_double)]
+
+
+class QueryResult(ctypes.Structure):
+    _fields_ = [("start_s", ctypes.c_double), ("done_s", ctypes.c_double), ("value", ctypes.c_double * 2),
+                ("zsum", ctypes.c_double), ("batch", ctypes.c_int), ("cached", ctypes.c_int)]
 DIST_NORMAL, DIST_UNIFORM = 0, 1
 
 _p = ctypes.c_void_p
@@ -72,6 +84,7 @@ SIGNATURES = {
     "qtnh_net_step": [_p, _u64, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "qtnh_net_run": [_p, _p, _dp, _u64],
     "qtnh_net_run_batch": [_p, _i, ctypes.POINTER(_p), ctypes.POINTER(_dp), _u64, ctypes.POINTER(_i)],
+    "qtnh_serve": [_p, _i, _p, _p, _u64, _i],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_zgemm": None,

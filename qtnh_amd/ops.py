@@ -443,3 +443,47 @@ def run_networks(nets, budget_bytes=1 << 40, device=None):
     waves = ctypes.c_int(0)
     check(lib().qtnh_net_run_batch(ctx, len(nets), hs, ps, budget_bytes, ctypes.byref(waves)), ctx)
     return [o[:_size(n.result_dims())] for o, n in zip(outs, nets)], waves.value
+
+
+def serve(queries, budget_bytes, policy=_lib.SERVE_BATCHED, device=None):
+    """Run the serving loop on a list of dicts (n_qubits, depth, seed, kind,
+    basis, arrival_s); returns one dict per query with start/done times,
+    latency (done - arrival), value, zsum, batch and cached."""
+    n = len(queries)
+    qs = (_lib.Query * max(n, 1))()
+    for i, q in enumerate(queries):
+        qs[i] = _lib.Query(q["n_qubits"], q["depth"], q["seed"], q["kind"], q.get("basis", 0), q["arrival_s"])
+    rs = (_lib.QueryResult * max(n, 1))()
+    ctx = context(device)
+    check(lib().qtnh_serve(ctx, n, ctypes.cast(qs, ctypes.c_void_p), ctypes.cast(rs, ctypes.c_void_p),
+                           budget_bytes, policy), ctx)
+    out = []
+    for i in range(n):
+        r = rs[i]
+        out.append({"start_s": r.start_s, "done_s": r.done_s, "latency_s": r.done_s - queries[i]["arrival_s"],
+                    "value": (r.value[0], r.value[1]), "zsum": r.zsum, "batch": r.batch, "cached": r.cached})
+    return out
+
+
+def poisson_queries(count, rate, seed, qubits=(10, 28), depth=(10, 30), pool=None, amp_fraction=0.5):
+    """Seeded open-loop workload: Poisson arrivals at `rate` per second, circuit
+    sizes uniform in the given ranges; with `pool`, circuits are drawn from that
+    many distinct specs (repeats hit resident states)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    specs = None
+    if pool:
+        specs = [(int(rng.integers(qubits[0], qubits[1] + 1)), int(rng.integers(depth[0], depth[1] + 1)),
+                  int(rng.integers(0, 2 ** 62))) for _ in range(pool)]
+    out = []
+    for _ in range(count):
+        t += float(rng.exponential(1.0 / rate)) if rate else 0.0
+        if specs:
+            n, d, s = specs[int(rng.integers(0, len(specs)))]
+        else:
+            n, d, s = (int(rng.integers(qubits[0], qubits[1] + 1)), int(rng.integers(depth[0], depth[1] + 1)),
+                       int(rng.integers(0, 2 ** 62)))
+        kind = _lib.QUERY_AMPLITUDE if rng.random() < amp_fraction else _lib.QUERY_EXPECT_Z
+        out.append({"n_qubits": n, "depth": d, "seed": s, "kind": kind,
+                    "basis": int(rng.integers(0, 2 ** n)), "arrival_s": t})
+    return out
