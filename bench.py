@@ -39,6 +39,15 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain device copy of the state first (HBM counter calibration)")
+    p.add_argument("--no-extra", action="store_true", help="skip the other configurations (C1, C3, C4, C5)")
+    p.add_argument("--only", default="", help="comma list of extra configurations to run (c1,c3c,c3g,c4,c5)")
+    p.add_argument("--circuit-rank", type=int, default=30)
+    p.add_argument("--circuit-depth", type=int, default=20)
+    p.add_argument("--gemm-n", type=int, default=4096)
+    p.add_argument("--networks", type=int, default=256)
+    p.add_argument("--network-max-rank", type=int, default=24)
+    p.add_argument("--serve-queries", type=int, default=48)
+    p.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, queries per second")
     return p.parse_args()
 
 
@@ -110,6 +119,189 @@ def measured_traffic(kernel):
         if name.startswith(_KERNEL_PREFIX[kernel]):
             return v["bytes"]
     return None
+
+
+def _pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q))
+
+
+def bench_c1(torch, ops, _lib):
+    """QFT-12 (reference examples/qft.cpp, tests/gen/qft.hpp): latency of one
+    circuit application to a resident 4096-amplitude state, one launch per
+    gate vs fused; the two results are compared bitwise."""
+    n = 12
+    c = ops.Circuit(n).qft()
+    out = {"qubits": n, "gates": c.info()[1]}
+    res = {}
+    for name, mode in (("per_gate", _lib.RUN_PER_GATE), ("fused", _lib.RUN_FUSED)):
+        s = ops.fill_(ops.empty(1 << n), 7)
+        ops.scale_(s, 1 / math.sqrt(ops.norm2(s)))
+        x0 = s.clone()
+        for _ in range(10):
+            s.copy_(x0)
+            c.run(s, mode)
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(200):
+            s.copy_(x0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            c.run(s, mode)
+            torch.cuda.synchronize()
+            lat.append((time.perf_counter() - t0) * 1e6)
+        res[name] = s
+        out[name] = {"p50_us": round(_pct(lat, 50), 1), "p99_us": round(_pct(lat, 99), 1)}
+    out["bitwise_equal"] = bool(torch.equal(torch.view_as_real(res["per_gate"]), torch.view_as_real(res["fused"])))
+    return out
+
+
+def bench_c3_circuit(torch, ops, _lib, n, depth, seed):
+    """Random circuit: every layer a Haar SU(2) on each qubit + CZ on a seeded
+    random pairing; one launch per gate vs fused LDS-tiled runs (bit-identical)."""
+    c = ops.Circuit(n).random(depth, seed)
+    _, nops, nruns = c.info()
+    N = 1 << n
+    a = ops.fill_(ops.empty(N), seed)
+    ops.scale_(a, 1 / math.sqrt(ops.norm2(a)))
+    b = a.clone()
+    out = {"qubits": n, "depth": depth, "gates": nops, "fused_runs": nruns,
+           "state_GiB": round(N * 16 / 2 ** 30, 2)}
+    for name, mode, st, passes in (("per_gate", _lib.RUN_PER_GATE, a, nops), ("fused", _lib.RUN_FUSED, b, nruns)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        c.run(st, mode)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        out[name] = {"ms": round(ms, 2), "passes": passes, "ms_per_pass": round(ms / passes, 3),
+                     "GB_per_s_per_pass": round(32.0 * N * passes / ms / 1e6, 1),
+                     "gate_GB_per_s": round(32.0 * N * nops / ms / 1e6, 1)}
+    out["speedup_fused"] = round(out["per_gate"]["ms"] / out["fused"]["ms"], 2)
+    out["bitwise_equal"] = bool(torch.equal(torch.view_as_real(a), torch.view_as_real(b)))
+    out["norm_after"] = ops.norm2(b)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_c3_gemm(torch, ops, _lib, n, seed):
+    """Two rank-2r tensors sharing r axes (M = N = K = n): FP64 complex GEMM vs
+    split precision (INT8 matrix cores, 6 slices); error vs the FP64 result."""
+    r = int(round(math.log2(n)))
+    a = ops.fill_(ops.empty(n * n), seed)
+    b = ops.fill_(ops.empty(n * n), seed + 1)
+    dims = [2] * (2 * r)
+    wires = [(r + i, i) for i in range(r)]
+    c64 = ops.empty(n * n)
+    cs = ops.empty(n * n)
+    flops = 8.0 * n ** 3
+    out = {"M": n, "N": n, "K": n, "GFLOP": flops / 1e9}
+    for name, prec, dst in (("fp64", _lib.PREC_FP64, c64), ("split", _lib.PREC_SPLIT, cs)):
+        ops.contract(a, dims, b, dims, wires, precision=prec, out=dst)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record()
+        for _ in range(reps):
+            _, _, eta = ops.contract(a, dims, b, dims, wires, precision=prec, out=dst)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": round(ms, 3), "TFLOP/s": round(flops / ms / 1e9, 2)}
+        if name == "split":
+            out[name]["eta_bound"] = eta
+    out["split_speedup"] = round(out["fp64"]["ms"] / out["split"]["ms"], 2)
+    out["split_rel_err"] = float(((cs - c64).norm() / c64.norm()).item())
+    out["fp64_frac_of_peak"] = round(out["fp64"]["TFLOP/s"] / 78.6, 3)
+    del a, b, c64, cs
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_c4(torch, ops, _lib, count, max_rank, seed):
+    """Amplitude networks of grid circuits (L x L, L in {4,5,6}, depth 8..16,
+    SU(2) + CZ), greedy seeded order; one network at a time vs one batch."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    nets, redrawn, s = [], 0, 0
+    while len(nets) < count:
+        L = int(rng.integers(4, 7))
+        d = int(rng.integers(8, 17))
+        s += 1
+        c = ops.Circuit(L * L).grid(L, L, d, seed * 1000 + s)
+        net = ops.Network.amplitude(c, rng.integers(0, 2, L * L))
+        try:
+            net.plan(s, max_rank)
+        except _lib.QtnhError:
+            redrawn += 1
+            continue
+        nets.append(net)
+    info = [n.info() for n in nets]
+    for n in nets[:4]:
+        n.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo = [n.run() for n in nets]
+    t1 = time.perf_counter()
+    ops.run_networks(nets[:4])
+    t2 = time.perf_counter()
+    batch, waves = ops.run_networks(nets)
+    t3 = time.perf_counter()
+    flops = sum(i["flops"] for i in info)
+    return {"networks": count, "redrawn_over_rank_cap": redrawn, "rank_cap": max_rank,
+            "max_rank": max(i["max_rank"] for i in info), "steps": sum(i["steps"] for i in info),
+            "GFLOP": round(flops / 1e9, 2), "GB_moved": round(sum(i["bytes"] for i in info) / 1e9, 2),
+            "one_at_a_time_ms": round((t1 - t0) * 1e3, 1), "batched_ms": round((t3 - t2) * 1e3, 1),
+            "speedup": round((t1 - t0) / (t3 - t2), 2), "waves": waves,
+            "batched_GFLOP_per_s": round(flops / (t3 - t2) / 1e9, 1),
+            "bitwise_equal": all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, batch))}
+
+
+def bench_c5(torch, ops, _lib, count, rate, seed):
+    """Serving: Poisson arrivals, circuits of 10..28 qubits and depth 10..30
+    (random SU(2) + CZ), half amplitude and half <Z> queries; sequential
+    one-launch-per-gate FP64 vs batched fused FP64 with resident states."""
+    import numpy as np
+    qs = ops.poisson_queries(count, rate, seed, qubits=(10, 28), depth=(10, 30))
+    out = {"queries": count, "rate_per_s": rate, "qubits": [10, 28], "depth": [10, 30]}
+    res = {}
+    for name, pol in (("sequential", _lib.SERVE_SEQUENTIAL), ("batched", _lib.SERVE_BATCHED)):
+        t0 = time.perf_counter()
+        r = ops.serve(qs, budget_bytes=200 << 30, policy=pol)
+        wall = time.perf_counter() - t0
+        lat = [x["latency_s"] * 1e3 for x in r]
+        res[name] = r
+        out[name] = {"p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
+                     "throughput_qps": round(count / max(x["done_s"] for x in r), 2), "wall_s": round(wall, 2)}
+    out["p99_speedup"] = round(out["sequential"]["p99_ms"] / out["batched"]["p99_ms"], 2)
+    out["throughput_speedup"] = round(out["batched"]["throughput_qps"] / out["sequential"]["throughput_qps"], 2)
+    out["answers_bitwise_equal"] = all(
+        np.array_equal(np.array(a["value"]).view(np.uint64), np.array(b["value"]).view(np.uint64))
+        for a, b in zip(res["sequential"], res["batched"]))
+    return out
+
+
+def extra_configs(args):
+    import torch
+
+    from qtnh_amd import _lib, ops
+
+    only = set(x for x in args.only.split(",") if x)
+    want = lambda k: not only or k in only  # noqa: E731
+    out = {}
+    if want("c1"):
+        out["C1_qft12_latency"] = bench_c1(torch, ops, _lib)
+    if want("c3c"):
+        out["C3_random_circuit"] = bench_c3_circuit(torch, ops, _lib, args.circuit_rank, args.circuit_depth, args.seed)
+    if want("c3g"):
+        out["C3_gemm"] = bench_c3_gemm(torch, ops, _lib, args.gemm_n, args.seed)
+    if want("c4"):
+        out["C4_batched_networks"] = bench_c4(torch, ops, _lib, args.networks, args.network_max_rank, args.seed)
+    if want("c5"):
+        out["C5_serving"] = bench_c5(torch, ops, _lib, args.serve_queries, args.serve_rate, args.seed)
+    return out
 
 
 def _cpu_model():
@@ -216,6 +408,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per},
         "wall_s_timed": round(wall, 3),
     }
+    del x, y
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        out["configs"] = extra_configs(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, ptup)
     print(json.dumps(out))

@@ -255,8 +255,10 @@ void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks
 void qtnh_tune_zgemm(int bn);
 /* Number of 7-bit slices (1..8, default 6) of QTNH_PREC_SPLIT contractions. */
 void qtnh_tune_split(int slices);
-/* Tile bits (3..13, default 12) of fused circuit runs. */
-void qtnh_tune_circuit(int tile_bits);
+/* Tile bits (3..13, default 12) of fused circuit runs, and how many of the
+ * lowest state bits every tile keeps (0..8, default 4; 2^low x 16 bytes are
+ * contiguous in each tile row).  Results do not depend on them. */
+void qtnh_tune_circuit(int tile_bits, int low_bits);
 
 #ifdef __cplusplus
 }

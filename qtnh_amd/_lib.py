@@ -61,7 +61,7 @@ SIGNATURES = {
     "qtnh_tune_permute": [_i, _i, _i, _i, _i, _i],
     "qtnh_tune_zgemm": [_i],
     "qtnh_tune_split": [_i],
-    "qtnh_tune_circuit": [_i],
+    "qtnh_tune_circuit": [_i, _i],
     "qtnh_circuit_create": [_i, ctypes.POINTER(_p)],
     "qtnh_circuit_destroy": [_p],
     "qtnh_circuit_gate": [_p, _i, _u16p, _dp],

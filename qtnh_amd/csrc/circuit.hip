@@ -36,7 +36,6 @@ namespace {
 
 typedef double double2_t __attribute__((ext_vector_type(2)));
 
-constexpr int kCircThreads = 256;
 constexpr int kMaxTileBits = 13;  // 2^13 amplitudes = 128 KiB of LDS
 constexpr int kMaxOpQubits = 3;
 
@@ -55,12 +54,15 @@ struct RunDesc {
   double2_t *psi;
   uint64_t first;       // first tile of this run in the launch's tile numbering
   uint64_t outer_mask;  // state bits outside the tile
-  uint64_t lo_mask;     // the tile's lowest min(tb, 8) state bits
-  uint64_t hi_mask;     // the tile's other state bits
+  uint64_t tile_mask;   // state bits of the tile
   uint32_t op_begin, op_end;
+  uint32_t coef_begin;  // first coefficient of op_begin (ops' coefficients are contiguous)
   int tb;               // tile bits
   int n_outer;          // popcount(outer_mask)
 };
+
+constexpr int kChunkOps = 32;     // ops staged in LDS at a time
+constexpr int kChunkCoef = 512;   // their coefficients (complex)
 
 // Parallel bit deposit: the low bits of v, in order, onto the set bits of mask.
 __device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask) {
@@ -80,9 +82,25 @@ __device__ __forceinline__ double2_t cmul2(double2_t a, double2_t b) {
   return double2_t{r, i};
 }
 
+// Group gi of a k-qubit op: tile index with zero bits inserted at the op's bits.
 template <int KB>
+__device__ __forceinline__ int group_base(int gi, const TileOp &op) {
+  int base = gi;
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    const int p = op.zpos[j];
+    base = ((base >> p) << (p + 1)) | (base & ((1 << p) - 1));
+  }
+  return base;
+}
+
+// Dense k-qubit op on a tile in LDS.  U groups per iteration: all their loads
+// are issued before any store (the groups are disjoint), so LDS latency
+// overlaps across groups.
+template <int KB, int NT>
 __device__ void gate_in_lds(double2_t *buf, int tb, const TileOp &op, const double2_t *g) {
   constexpr int K = 1 << KB;
+  constexpr int U = KB == 3 ? 1 : 2;
   int comb[K];
 #pragma unroll
   for (int c = 0; c < K; ++c) {
@@ -93,50 +111,74 @@ __device__ void gate_in_lds(double2_t *buf, int tb, const TileOp &op, const doub
     comb[c] = off;
   }
   const int groups = 1 << (tb - KB);
-  for (int gi = threadIdx.x; gi < groups; gi += kCircThreads) {
-    // 3-qubit gates: re-read the 64 coefficients from LDS in every group
-    // rather than holding them in 256 registers (a compiler barrier, no code).
-    if (KB == 3) asm volatile("" ::: "memory");
-    int base = gi;
+  for (int g0 = threadIdx.x; g0 < groups; g0 += NT * U) {
+    // Large gates (and 2-qubit gates in 1024-thread blocks, which have 128
+    // registers): re-read the coefficients from LDS in every iteration rather
+    // than holding them in registers (a compiler barrier, no code).
+    if (KB == 3 || (KB == 2 && NT == 1024)) asm volatile("" ::: "memory");
+    int base[U];
+    bool live[U];
+    double2_t in[U][K];
 #pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      const int p = op.zpos[j];
-      base = ((base >> p) << (p + 1)) | (base & ((1 << p) - 1));
+    for (int u = 0; u < U; ++u) {
+      const int gi = g0 + u * NT;
+      live[u] = gi < groups;
+      base[u] = group_base<KB>(live[u] ? gi : g0, op);
+#pragma unroll
+      for (int c = 0; c < K; ++c) in[u][c] = buf[base[u] + comb[c]];
     }
-    double2_t in[K];
 #pragma unroll
-    for (int c = 0; c < K; ++c) in[c] = buf[base + comb[c]];
+    for (int u = 0; u < U; ++u) {
+      if (!live[u]) continue;
 #pragma unroll
-    for (int o = 0; o < K; ++o) {
-      double2_t acc = {0.0, 0.0};
+      for (int o = 0; o < K; ++o) {
+        double2_t acc = {0.0, 0.0};
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const double2_t p = cmul2(in[i], g[i * K + o]);
-        acc.x = acc.x + p.x;
-        acc.y = acc.y + p.y;
+        for (int i = 0; i < K; ++i) {
+          const double2_t p = cmul2(in[u][i], g[i * K + o]);
+          acc.x = acc.x + p.x;
+          acc.y = acc.y + p.y;
+        }
+        buf[base[u] + comb[o]] = acc;
       }
-      buf[base + comb[o]] = acc;
     }
   }
 }
 
-template <int KB>
+template <int KB, int NT>
 __device__ void diag_in_lds(double2_t *buf, int tb, const TileOp &op, const double2_t *d) {
+  constexpr int U = 4;
   const int T = 1 << tb;
-  for (int l = threadIdx.x; l < T; l += kCircThreads) {
-    int c = 0;
+  for (int l0 = threadIdx.x; l0 < T; l0 += NT * U) {
+    double2_t v[U];
+    int c[U];
 #pragma unroll
-    for (int j = 0; j < KB; ++j) c = (c << 1) | ((l >> op.pos[j]) & 1);
-    buf[l] = cmul2(buf[l], d[c]);
+    for (int u = 0; u < U; ++u) {
+      const int l = l0 + u * NT;
+      c[u] = 0;
+#pragma unroll
+      for (int j = 0; j < KB; ++j) c[u] = (c[u] << 1) | ((l >> op.pos[j]) & 1);
+      v[u] = l < T ? buf[l] : double2_t{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int l = l0 + u * NT;
+      if (l < T) buf[l] = cmul2(v[u], d[c[u]]);
+    }
   }
 }
 
 // One workgroup per tile (persistent over tiles); runs[] sorted by `first`.
-template <int TBMAX>
-__global__ void __launch_bounds__(kCircThreads, (TBMAX >= 13 ? 1 : 2)) circuit_tile_kernel(const RunDesc *__restrict__ runs, int nruns,
-                                                                    uint64_t total_tiles) {
+// Ops and their coefficients are staged into LDS in chunks; each op is
+// followed by one barrier.
+template <int TBMAX, int NT>
+__global__ void __launch_bounds__(NT, (NT == 256 && TBMAX < 13) ? 2 : 1)
+    circuit_tile_kernel(const RunDesc *__restrict__ runs, int nruns, uint64_t total_tiles) {
+  constexpr int LB = NT == 1024 ? 10 : 8;  // tile bits taken from the thread index
   __shared__ double2_t buf[1 << TBMAX];
-  __shared__ double2_t gco[1 << (2 * kMaxOpQubits)];  // coefficients of the current op
+  __shared__ double2_t cbuf[kChunkCoef];
+  __shared__ TileOp obuf[kChunkOps];
+  __shared__ uint64_t hi_off[(1 << TBMAX) / NT + 1];
   for (uint64_t gt = blockIdx.x; gt < total_tiles; gt += gridDim.x) {
     int lo = 0, hi = nruns - 1;
     while (lo < hi) {
@@ -149,33 +191,60 @@ __global__ void __launch_bounds__(kCircThreads, (TBMAX >= 13 ? 1 : 2)) circuit_t
     const RunDesc &R = runs[lo];
     const uint64_t t = gt - R.first;
     const int T = 1 << R.tb;
-    // Tile element l = thread + kCircThreads j: its low 8 tile bits are the
-    // thread's, the others come from j.
-    double2_t *psi = R.psi + deposit(t, R.outer_mask);
-    const uint64_t lo_off = deposit(threadIdx.x, R.lo_mask);
-    for (int l = threadIdx.x, j = 0; l < T; l += kCircThreads, ++j) buf[l] = psi[lo_off | deposit(j, R.hi_mask)];
-    __syncthreads();
-    for (uint32_t o = R.op_begin; o < R.op_end; ++o) {
-      const TileOp op = R.ops[o];
-      const int nco = op.kind == 1 ? (1 << op.kb) : (1 << (2 * op.kb));
-      if ((int)threadIdx.x < nco) gco[threadIdx.x] = R.coef[op.coef + threadIdx.x];
-      __syncthreads();
-      if (op.kind == 1) {
-        if (op.kb == 1)
-          diag_in_lds<1>(buf, R.tb, op, gco);
-        else if (op.kb == 2)
-          diag_in_lds<2>(buf, R.tb, op, gco);
-        else
-          diag_in_lds<3>(buf, R.tb, op, gco);
-      } else if (op.kb == 1)
-        gate_in_lds<1>(buf, R.tb, op, gco);
-      else if (op.kb == 2)
-        gate_in_lds<2>(buf, R.tb, op, gco);
-      else
-        gate_in_lds<3>(buf, R.tb, op, gco);
-      __syncthreads();
+    const int nj = (T + NT - 1) / NT;
+    // Tile element l = thread + NT j: its low LB tile bits are the thread's,
+    // the others come from j (table hi_off).
+    uint64_t lo_mask = 0, rest = R.tile_mask;
+    for (int b = 0; b < LB && rest; ++b) {
+      const uint64_t low = rest & (0 - rest);
+      lo_mask |= low;
+      rest ^= low;
     }
-    for (int l = threadIdx.x, j = 0; l < T; l += kCircThreads, ++j) psi[lo_off | deposit(j, R.hi_mask)] = buf[l];
+    if ((int)threadIdx.x < nj) hi_off[threadIdx.x] = deposit(threadIdx.x, rest);
+    double2_t *psi = R.psi + deposit(t, R.outer_mask);
+    const uint64_t lo_off = deposit(threadIdx.x, lo_mask);
+    __syncthreads();
+    for (int l = threadIdx.x, j = 0; l < T; l += NT, ++j) buf[l] = psi[lo_off | hi_off[j]];
+    uint32_t o = R.op_begin;
+    uint32_t cbase = R.coef_begin;
+    while (o < R.op_end) {
+      // Stage the next ops whose coefficients fit the chunk.
+      __syncthreads();  // previous chunk's ops are done with obuf / cbuf
+      uint32_t oe = o, ce = cbase;
+      while (oe < R.op_end && oe - o < (uint32_t)kChunkOps) {
+        const TileOp &x = R.ops[oe];
+        const uint32_t nco = x.kind == 1 ? (1u << x.kb) : (1u << (2 * x.kb));
+        if (x.coef + nco - cbase > (uint32_t)kChunkCoef) break;
+        ce = x.coef + nco;
+        ++oe;
+      }
+      for (uint32_t i = threadIdx.x; i < oe - o; i += NT) obuf[i] = R.ops[o + i];
+      for (uint32_t i = threadIdx.x; i < ce - cbase; i += NT) cbuf[i] = R.coef[cbase + i];
+      __syncthreads();
+      for (uint32_t i = 0; i < oe - o; ++i) {
+        const TileOp &op = obuf[i];
+        const double2_t *g = cbuf + (op.coef - cbase);
+        if (op.kind == 1) {
+          if (op.kb == 1)
+            diag_in_lds<1, NT>(buf, R.tb, op, g);
+          else if (op.kb == 2)
+            diag_in_lds<2, NT>(buf, R.tb, op, g);
+          else
+            diag_in_lds<3, NT>(buf, R.tb, op, g);
+        } else if (op.kb == 1) {
+          gate_in_lds<1, NT>(buf, R.tb, op, g);
+        } else if (op.kb == 2) {
+          gate_in_lds<2, NT>(buf, R.tb, op, g);
+        } else {
+          gate_in_lds<3, NT>(buf, R.tb, op, g);
+        }
+        __syncthreads();
+      }
+      o = oe;
+      cbase = ce;
+    }
+    if (R.op_begin == R.op_end) __syncthreads();
+    for (int l = threadIdx.x, j = 0; l < T; l += NT, ++j) psi[lo_off | hi_off[j]] = buf[l];
     __syncthreads();
   }
 }
@@ -285,12 +354,14 @@ struct qtnh_circuit_s {
     std::vector<int> tbits;  // ascending state bits
   };
   std::vector<Run> runs;
+  std::vector<TileOp> tops;  // host copy of the compiled ops
 };
 
 namespace qtnh_hip {
 namespace {
 
 int g_tile_bits = 12;
+int g_low_bits = 4;  // lowest state bits in every tile: 2^4 x 16 B = 256 B contiguous rows
 
 uint64_t mask_of(const qtnh_circuit_s &c, const qtnh_circuit_s::Op &op) {
   uint64_t m = 0;
@@ -303,7 +374,8 @@ uint64_t mask_of(const qtnh_circuit_s &c, const qtnh_circuit_s::Op &op) {
 void plan_runs(qtnh_circuit_s &c, int tb) {
   c.runs.clear();
   const int tbe = std::min(tb, c.n);
-  uint64_t cur = 0;
+  const uint64_t low = (1ull << std::min(std::min(g_low_bits, tbe - 3 > 0 ? tbe - 3 : 0), c.n)) - 1;
+  uint64_t cur = low;
   uint32_t begin = 0;
   auto close = [&](uint32_t end) {
     if (end == begin) return;
@@ -321,7 +393,7 @@ void plan_runs(qtnh_circuit_s &c, int tb) {
     if (__builtin_popcountll(cur | m) > tbe) {
       close(i);
       begin = i;
-      cur = 0;
+      cur = low;
     }
     cur |= m;
   }
@@ -337,7 +409,8 @@ void free_compiled(qtnh_circuit_s &c) {
 }
 
 qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
-  if (c.compiled_tb == g_tile_bits && c.device == ctx->device && c.compiled_ops == c.ops.size()) return QTNH_OK;
+  if (c.compiled_tb == g_tile_bits * 16 + g_low_bits && c.device == ctx->device && c.compiled_ops == c.ops.size())
+    return QTNH_OK;
   if (c.d_ops || c.d_coef) {
     qtnh_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "circuit recompile sync");
     if (st) return st;
@@ -379,7 +452,8 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
     free_compiled(c);
     return st;
   }
-  c.compiled_tb = g_tile_bits;
+  c.tops = tops;
+  c.compiled_tb = g_tile_bits * 16 + g_low_bits;
   c.device = ctx->device;
   c.compiled_ops = c.ops.size();
   return QTNH_OK;
@@ -394,15 +468,11 @@ RunDesc make_desc(const qtnh_circuit_s &c, const qtnh_circuit_s::Run &r, c128 *p
   d.first = first;
   d.op_begin = r.begin;
   d.op_end = r.end;
+  d.coef_begin = r.begin < c.tops.size() ? c.tops[r.begin].coef : 0;
   d.tb = (int)r.tbits.size();
   uint64_t tile = 0;
-  for (size_t j = 0; j < r.tbits.size(); ++j) {
-    tile |= 1ull << r.tbits[j];
-    if (j < 8)
-      d.lo_mask |= 1ull << r.tbits[j];
-    else
-      d.hi_mask |= 1ull << r.tbits[j];
-  }
+  for (size_t j = 0; j < r.tbits.size(); ++j) tile |= 1ull << r.tbits[j];
+  d.tile_mask = tile;
   const uint64_t all = c.n == 64 ? ~0ull : ((1ull << c.n) - 1);
   d.outer_mask = all & ~tile;
   d.n_outer = __builtin_popcountll(d.outer_mask);
@@ -431,17 +501,26 @@ qtnh_status launch_run_groups(qtnh_ctx_t ctx, const std::vector<RunDesc> &descs,
       tbmax = std::max(tbmax, descs[i].tb);
       total = descs[i].first + desc_tiles(descs[i]);
     }
-    const unsigned per_cu = tbmax <= 10 ? 8 : tbmax == 11 ? 4 : tbmax == 12 ? 2 : 1;
-    const unsigned blocks = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * per_cu);
     const int nr = (int)(e - b);
-    if (tbmax <= 10)
-      hipLaunchKernelGGL(circuit_tile_kernel<10>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
-    else if (tbmax == 11)
-      hipLaunchKernelGGL(circuit_tile_kernel<11>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
-    else if (tbmax == 12)
-      hipLaunchKernelGGL(circuit_tile_kernel<12>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
-    else
-      hipLaunchKernelGGL(circuit_tile_kernel<13>, dim3(blocks), dim3(kCircThreads), 0, ctx->stream, dd + b, nr, total);
+    // Few tiles: 1024 threads per tile; many: 256 threads, several tiles per CU.
+    const bool wide = total < (uint64_t)ctx->num_cus;
+    const unsigned per_cu = wide ? 1 : tbmax <= 10 ? 8 : tbmax == 11 ? 4 : tbmax == 12 ? 2 : 1;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * per_cu);
+#define QTNH_TILE_LAUNCH(TBM)                                                                                       \
+  if (wide)                                                                                                       \
+    hipLaunchKernelGGL((circuit_tile_kernel<TBM, 1024>), dim3(blocks), dim3(1024), 0, ctx->stream, dd + b, nr, total); \
+  else                                                                                                            \
+    hipLaunchKernelGGL((circuit_tile_kernel<TBM, 256>), dim3(blocks), dim3(256), 0, ctx->stream, dd + b, nr, total);
+    if (tbmax <= 10) {
+      QTNH_TILE_LAUNCH(10)
+    } else if (tbmax == 11) {
+      QTNH_TILE_LAUNCH(11)
+    } else if (tbmax == 12) {
+      QTNH_TILE_LAUNCH(12)
+    } else {
+      QTNH_TILE_LAUNCH(13)
+    }
+#undef QTNH_TILE_LAUNCH
     st = check_hip(ctx, hipGetLastError(), "circuit tile launch");
     if (st) return st;
   }
@@ -561,7 +640,10 @@ using namespace qtnh_hip;
 
 extern "C" {
 
-void qtnh_tune_circuit(int tile_bits) { g_tile_bits = std::max(3, std::min(tile_bits, kMaxTileBits)); }
+void qtnh_tune_circuit(int tile_bits, int low_bits) {
+  g_tile_bits = std::max(3, std::min(tile_bits, kMaxTileBits));
+  g_low_bits = std::max(0, std::min(low_bits, 8));
+}
 
 qtnh_status qtnh_circuit_create(int n_qubits, qtnh_circuit_t *out) {
   if (!out) return QTNH_EINVAL;

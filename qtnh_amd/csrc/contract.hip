@@ -237,13 +237,23 @@ qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const 
   return check_hip(ctx, hipGetLastError(), "contract_direct_general launch");
 }
 
-// Route choice: the direct kernel re-reads each operand element once per
-// result element that uses it, which is cheap while the closed size or one
-// open side is small (gates, vectors, tiny tensors).
+// Route choice: the direct kernel sums K operand pairs per result element in
+// one thread, with no reuse, which is cheap while the sums are short (gates)
+// or the whole contraction is small and its sums stay short.
 static bool use_direct(const Plan &P) {
   if (P.nw > kMaxClosed || P.rC > kMaxOut) return false;
-  if (P.K <= 16 || P.M <= 4 || P.N <= 4) return true;
-  return P.M * P.N * P.K <= (1ull << 18);
+  return P.K <= 16 || (P.M * P.N * P.K <= (1ull << 20) && P.K <= 1024);
+}
+
+// Few outputs, long sums (inner products, closing steps of a network): the
+// sums are split into ordered chunks over many workgroups.
+static bool use_splitk(const Plan &P) { return P.M * P.N <= (1ull << 14); }
+
+// Chunks per output of the split-K route; independent of the device.
+static uint64_t splitk_chunks(uint64_t MN, uint64_t K) {
+  const uint64_t want = (2048 + MN - 1) / MN;
+  const uint64_t most = (K + 4095) / 4096;
+  return std::max<uint64_t>(1, std::min(want, most));
 }
 
 ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
@@ -256,10 +266,14 @@ ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t 
   R.direct = use_direct(P);
   R.ws_bytes = 0;
   if (!R.direct) {
-    // Same layout as launch_contract below: permuted A, permuted B, natural C, split scratch.
+    // Same layout as launch_contract below: permuted A, permuted B, natural C,
+    // then split scratch or split-K partials.
     auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
     R.ws_bytes = pad(P.M * P.K) + pad(P.K * P.N) + pad(P.nC);
-    if (precision == QTNH_PREC_SPLIT) R.ws_bytes += (split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull;
+    if (use_splitk(P))
+      R.ws_bytes += pad(P.M * P.N * splitk_chunks(P.M * P.N, P.K));
+    else if (precision == QTNH_PREC_SPLIT)
+      R.ws_bytes += (split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull;
   }
   return R;
 }
@@ -273,16 +287,18 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     if (err_out) *err_out = 0.0;
     return launch_contract_direct(ctx, A, rA, dA, B, rB, dB, nw, wires, r1, r2, C);
   }
-  const bool split = precision == QTNH_PREC_SPLIT;
+  const bool splitk = use_splitk(P);
+  const bool split = precision == QTNH_PREC_SPLIT && !splitk;
 
   // GEMM route.  A' = A permuted to [open1, closed]; B' = B permuted to
-  // [closed, open2] (FP64) or [open2, closed] (split, which reads B by columns).
+  // [closed, open2] (FP64 GEMM) or [open2, closed] (split precision and
+  // split-K, which read B by columns).
   std::vector<uint16_t> pA(rA), pB(rB), pC(P.rC);
   int q = 0;
   for (int i : P.open1) pA[i] = (uint16_t)q++;
   for (int i : P.cA) pA[i] = (uint16_t)q++;
   q = 0;
-  if (split) {
+  if (split || splitk) {
     for (int i : P.open2) pB[i] = (uint16_t)q++;
     for (int i : P.cB) pB[i] = (uint16_t)q++;
   } else {
@@ -298,7 +314,10 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
   }
   const uint64_t nA = P.M * P.K, nB = P.K * P.N;
   auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
-  const uint64_t sbytes = split ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull) : 0;
+  const uint64_t kchunks = splitk ? splitk_chunks(P.M * P.N, P.K) : 0;
+  const uint64_t sbytes = split    ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull)
+                          : splitk ? pad(P.M * P.N * kchunks)
+                                   : 0;
   const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC)) + sbytes;
   char *ws = nullptr;
   if (bytes && ext_ws) {
@@ -334,7 +353,10 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     off += pad(P.nC);
   }
   qtnh_status st;
-  if (split) {
+  if (splitk) {
+    st = launch_zgemm_splitk(ctx, Ap, Bp, Cn, P.M, P.N, P.K, kchunks, (c128 *)(ws + off));
+    if (err_out) *err_out = 0.0;
+  } else if (split) {
     st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + off, sbytes, err_out);
   } else {
     st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N);

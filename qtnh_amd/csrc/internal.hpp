@@ -100,6 +100,12 @@ qtnh_status launch_scale(qtnh_ctx_t ctx, c128 *x, uint64_t n, double re, double 
 qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N,
                              uint64_t K, uint64_t lda, uint64_t ldb, uint64_t ldc);
 
+// C[MxN] = A[MxK] * Bt[NxK]^T for few outputs: each sum in `chunks` ordered
+// chunks over separate workgroups (partials: M*N*chunks elements), then summed
+// in chunk order.
+qtnh_status launch_zgemm_splitk(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
+                                uint64_t K, uint64_t chunks, c128 *partials);
+
 // Split-precision complex GEMM C[MxN] = A[MxK] * Bt[NxK]^T (all contiguous,
 // row-major) on INT8 matrix cores; scratch of split_scratch_bytes(M, N, K).
 qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,

@@ -134,7 +134,66 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
       }
 }
 
+constexpr int kSkThreads = 256;
+
+// Block (output mn, chunk s): ordered per-thread sums, fixed tree in LDS.
+__global__ void __launch_bounds__(kSkThreads) splitk_partial_kernel(const double2_t *__restrict__ A,
+                                                                    const double2_t *__restrict__ Bt, uint64_t N,
+                                                                    uint64_t K, uint64_t kc, uint64_t S,
+                                                                    double2_t *__restrict__ part) {
+  __shared__ double2_t red[kSkThreads];
+  const uint64_t mn = blockIdx.x / S, s = blockIdx.x % S;
+  const uint64_t m = mn / N, n = mn % N;
+  const double2_t *a = A + m * K, *b = Bt + n * K;
+  const uint64_t k1 = std::min(K, (s + 1) * kc);
+  double re = 0.0, im = 0.0;
+  for (uint64_t k = s * kc + threadIdx.x; k < k1; k += kSkThreads) {
+    const double2_t x = a[k], y = b[k];
+    double pr, pi;
+    cmul_parts(x.x, x.y, y.x, y.y, &pr, &pi);
+    re = re + pr;
+    im = im + pi;
+  }
+  red[threadIdx.x] = double2_t{re, im};
+  __syncthreads();
+  for (int w = kSkThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double2_t u = red[threadIdx.x], v = red[threadIdx.x + w];
+      red[threadIdx.x] = double2_t{u.x + v.x, u.y + v.y};
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void splitk_final_kernel(const double2_t *__restrict__ part, uint64_t MN, uint64_t S,
+                                    double2_t *__restrict__ C) {
+  for (uint64_t mn = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; mn < MN; mn += (uint64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (uint64_t s = 0; s < S; ++s) {
+      const double2_t v = part[mn * S + s];
+      re = re + v.x;
+      im = im + v.y;
+    }
+    C[mn] = double2_t{re, im};
+  }
+}
+
 }  // namespace
+
+qtnh_status launch_zgemm_splitk(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
+                                uint64_t K, uint64_t chunks, c128 *partials) {
+  const uint64_t MN = M * N;
+  const uint64_t kc = ((K + chunks - 1) / chunks + kSkThreads - 1) / kSkThreads * kSkThreads;
+  if (MN * chunks > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "splitk: grid too large");
+  hipLaunchKernelGGL(splitk_partial_kernel, dim3((unsigned)(MN * chunks)), dim3(kSkThreads), 0, ctx->stream,
+                     (const double2_t *)A, (const double2_t *)Bt, N, K, kc, chunks, (double2_t *)partials);
+  qtnh_status st = check_hip(ctx, hipGetLastError(), "splitk launch");
+  if (st) return st;
+  hipLaunchKernelGGL(splitk_final_kernel, dim3(grid_for(MN, 256, 4096)), dim3(256), 0, ctx->stream,
+                     (const double2_t *)partials, MN, chunks, (double2_t *)C);
+  return check_hip(ctx, hipGetLastError(), "splitk final launch");
+}
 
 // Block width: 0 picks by size (64 x 128 tiles run at occupancy 1 and win
 // while 64 x 64 tiles would number under 2048, about 8 per compute unit;

@@ -228,9 +228,10 @@ def tune_split(slices=6):
     lib().qtnh_tune_split(slices)
 
 
-def tune_circuit(tile_bits=12):
-    """Tile bits (3..13) of fused circuit runs (process-wide)."""
-    lib().qtnh_tune_circuit(tile_bits)
+def tune_circuit(tile_bits=12, low_bits=4):
+    """Tile bits (3..13) of fused circuit runs and the lowest state bits every
+    tile keeps (0..8), process-wide; results do not depend on them."""
+    lib().qtnh_tune_circuit(tile_bits, low_bits)
 
 
 class Circuit:

@@ -8,7 +8,8 @@ Checked against the oracle (small sizes) and against FP64 complex GEMM
   * the error falls by about 2^-7 per extra slice;
   * rows and columns of very different magnitude, zero rows, ragged
     (non-power-of-two) dims and placements all keep the bound;
-  * problems small enough for the direct route stay exact FP64 (err 0).
+  * problems small enough for the direct route, and few-output long sums,
+    stay exact FP64 (err 0).
 """
 import numpy as np
 import pytest
@@ -74,7 +75,7 @@ def split_gemm(A, B, slices=6):
     return host(c).reshape(M, N), err
 
 
-@pytest.mark.parametrize("m,k,n", [(64, 128, 64), (128, 256, 64), (256, 128, 512), (32, 1024, 32)])
+@pytest.mark.parametrize("m,k,n", [(64, 256, 512), (128, 256, 256), (256, 128, 512), (64, 1024, 512)])
 def test_split_matches_oracle(m, k, n):
     A = rnd((m, k), 1)
     B = rnd((k, n), 2)
@@ -106,7 +107,7 @@ def test_split_large_vs_torch_fp64(n):
 
 
 def test_split_error_falls_with_slices():
-    A = rnd((128, 512), 3)
+    A = rnd((256, 512), 3)
     B = rnd((512, 128), 4)
     ref = A @ B
     errs = []
@@ -122,8 +123,8 @@ def test_split_error_falls_with_slices():
 
 def test_split_wide_dynamic_range_and_zero_rows():
     rng = np.random.default_rng(9)
-    A = rnd((128, 256), 5) * np.exp2(rng.integers(-300, 300, 128))[:, None]
-    B = rnd((256, 64), 6) * np.exp2(rng.integers(-300, 300, 64))[None, :]
+    A = rnd((256, 256), 5) * np.exp2(rng.integers(-300, 300, 256))[:, None]
+    B = rnd((256, 128), 6) * np.exp2(rng.integers(-300, 300, 128))[None, :]
     A[7] = 0
     B[:, 3] = 0
     A[9, 5] = 1e-310  # a subnormal in an otherwise zero row
@@ -139,8 +140,8 @@ def test_split_wide_dynamic_range_and_zero_rows():
 
 
 @pytest.mark.parametrize("dA,dB,wires", [
-    ([30, 40, 7], [7, 40, 5, 9], [(1, 1), (2, 0)]),      # ragged K = 280, not a multiple of 32
-    ([25, 6, 7, 8], [8, 6, 33], [(3, 0), (1, 1)]),       # non-adjacent wired axes
+    ([60, 40, 7], [7, 40, 5, 59], [(1, 1), (2, 0)]),     # ragged K = 280, not a multiple of 32
+    ([25, 6, 7, 8], [8, 6, 133], [(3, 0), (1, 1)]),      # non-adjacent wired axes
     ([2] * 14, [2] * 13, [(0, 8), (3, 2), (13, 0), (5, 4), (7, 6)]),
 ])
 def test_split_ragged_and_scattered_wires(dA, dB, wires):
@@ -155,17 +156,28 @@ def test_split_ragged_and_scattered_wires(dA, dB, wires):
 
 
 def test_split_placement_matches_fp64():
-    dA, dB = [64, 16, 32], [32, 16, 4, 64]
+    dA, dB = [64, 16, 32], [32, 16, 8, 64]
     wires = [(1, 1), (2, 0)]
     A = rnd(int(np.prod(dA)), 21)
     B = rnd(int(np.prod(dB)), 22)
     r1, r2 = [1, _lib.X, _lib.X], [_lib.X, _lib.X, 2, 0]
     cs, dcs, eta = ops.contract(dev(A), dA, dev(B), dB, wires, r1, r2, precision=SPLIT)
     cf, dcf, e0 = ops.contract(dev(A), dA, dev(B), dB, wires, r1, r2, precision=FP64)
-    assert dcs == dcf == [64, 64, 4]
+    assert dcs == dcf == [64, 64, 8]
     assert e0 == 0.0
     rel = np.linalg.norm(host(cs) - host(cf)) / np.linalg.norm(host(cf))
     assert rel <= 1e-10, rel
+
+
+def test_split_few_outputs_is_exact_fp64():
+    A = rnd((1, 8192), 41)
+    B = rnd((8192, 4), 42)
+    dA = [2] * 13
+    dB = [2] * 13 + [4]
+    cs, _, eta = ops.contract(dev(A), dA, dev(B), dB, [(i, i) for i in range(13)], precision=SPLIT)
+    cf, _, _ = ops.contract(dev(A), dA, dev(B), dB, [(i, i) for i in range(13)], precision=FP64)
+    assert eta == 0.0
+    assert np.array_equal(host(cs).view(np.uint64), host(cf).view(np.uint64))
 
 
 def test_split_small_problem_is_exact_fp64():
