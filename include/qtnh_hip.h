@@ -116,7 +116,7 @@ qtnh_status qtnh_diag_apply_c128(qtnh_ctx_t ctx, void *state, int rank, const ui
  * axes).  Small problems use a kernel that keeps the reference summation
  * order; large ones permute to matrices and run a complex GEMM on MFMA.
  * precision QTNH_PREC_FP64 (the drop-in) or QTNH_PREC_SPLIT: GEMM-sized
- * problems then run as exact INT8 matrix-core products of 7-bit slices of A
+ * problems then run as exact INT8 matrix-core products of 8-bit slices of A
  * and B recombined in FP64 (finite inputs only).  err_out (may be NULL)
  * receives eta of the componentwise a-priori bound
  *   |C_ij - C~_ij| <= eta * max_k|A_ik| * max_k|B_kj|   (max over re and im)
@@ -253,7 +253,7 @@ void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks
 /* Block width (64 or 128 complex columns, 0: chosen by size) of the FP64
  * complex GEMM. */
 void qtnh_tune_zgemm(int bn);
-/* Number of 7-bit slices (1..8, default 6) of QTNH_PREC_SPLIT contractions. */
+/* Number of 8-bit slices (1..8, default 5) of QTNH_PREC_SPLIT contractions. */
 void qtnh_tune_split(int slices);
 /* Tile bits (3..13, default 12) of fused circuit runs, and how many of the
  * lowest state bits every tile keeps (0..8, default 4; 2^low x 16 bytes are

@@ -8,14 +8,15 @@
 // so that C_e (M x 2N) is C's natural layout.
 //
 // Each row of A_e (and each column pair of B_e) is scaled by a power of two,
-// x = 2^e R with |R| < 1, and R is cut into S signed 7-bit slices
-//   R = sum_p q_p 2^-(6 + 7p) + r,  |q_p| <= 64,  |r| <= 2^-7S,
-// every step exact in binary64.  Products of slices are exact in int32
-// (v_mfma_i32_16x16x64_i8).  The kernel keeps one int32 accumulator per
-// diagonal d = p + q < S (the triangular truncation) and recombines
-//   C_ij = 2^(ea_i + eb_j) sum_d 2^-(12 + 7d) acc_d
-// in FP64 at the end.  With S = 6 the relative error on random complex
-// inputs is ~4e-12 (21 slice products instead of one FP64 GEMM).
+// x = 2^e R with |R| < 1.  Y = rint(R 2^(8S-2)) is an integer of at most
+// 8S-2 bits, written exactly as S balanced base-256 digits in [-128, 127]:
+//   R = sum_p q_p 2^-(6 + 8p) + r,  |r| <= 2^-(8S-1).
+// Products of digits are exact in int32 (v_mfma_i32_16x16x64_i8).  The
+// kernel keeps one int32 accumulator per diagonal d = p + q < S (the
+// triangular truncation) and recombines
+//   C_ij = 2^(ea_i + eb_j) sum_d 2^-(12 + 8d) acc_d
+// in FP64 at the end.  Relative error on Gaussian complex inputs: about
+// 7e-11 with S = 5 (15 digit products), 2e-13 with S = 6 (21).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -60,46 +61,184 @@ __global__ void __launch_bounds__(kThreads) split_rows_kernel(const c128 *__rest
       expo[2 * row + 1] = e;
     }
   }
+  // Balanced base-256 digits of rint(v 2^(8S-2-e)), most significant first.
+  auto digits = [&](double v, int8_t *d) {
+    long long y = (long long)rint(ldexp(v, 8 * S - 2 - e));
+#pragma unroll
+    for (int p = kMaxSlices - 1; p >= 0; --p) {
+      if (p >= S) continue;
+      const long long t = ((y + 128) & 255) - 128;
+      d[p] = (int8_t)t;
+      y = (y - t) >> 8;  // exact: y - t is a multiple of 256
+    }
+  };
+  auto pack = [](int8_t lo, int8_t hi) { return (uint16_t)((uint8_t)lo | ((uint16_t)(uint8_t)hi << 8)); };
   for (uint64_t k = threadIdx.x; k < K; k += kThreads) {
     const c128 v = x[k];
-    double r0 = ldexp(v.re, -e), r1 = ldexp(v.im, -e);
-    for (int p = 0; p < S; ++p) {
-      const int sh = 6 + 7 * p;
-      const double q0 = rint(ldexp(r0, sh)), q1 = rint(ldexp(r1, sh));
-      r0 -= ldexp(q0, -sh);
-      r1 -= ldexp(q1, -sh);
-      const int8_t a0 = (int8_t)q0, a1 = (int8_t)q1;
+    int8_t dr[kMaxSlices], di[kMaxSlices], dn[kMaxSlices];
+    digits(v.re, dr);
+    digits(v.im, di);
+    if (mode == 1) digits(-v.im, dn);
+#pragma unroll
+    for (int p = 0; p < kMaxSlices; ++p) {
+      if (p >= S) continue;
       int8_t *o = out + (uint64_t)p * plane;
       if (mode == 0) {
-        o[row * ldo + 2 * k] = a0;
-        o[row * ldo + 2 * k + 1] = a1;
+        *reinterpret_cast<uint16_t *>(o + row * ldo + 2 * k) = pack(dr[p], di[p]);
       } else {
-        o[(2 * row) * ldo + 2 * k] = a0;
-        o[(2 * row) * ldo + 2 * k + 1] = (int8_t)(-a1);
-        o[(2 * row + 1) * ldo + 2 * k] = a1;
-        o[(2 * row + 1) * ldo + 2 * k + 1] = a0;
+        *reinterpret_cast<uint16_t *>(o + (2 * row) * ldo + 2 * k) = pack(dr[p], dn[p]);
+        *reinterpret_cast<uint16_t *>(o + (2 * row + 1) * ldo + 2 * k) = pack(di[p], dr[p]);
       }
     }
   }
 }
 
-// Block tile: 128 real rows x 64 real columns, 8 waves of 32 x 32.  Operand
-// panels (S slices x rows x 64 bytes of depth) are staged through LDS, two
-// stages when they fit (S <= 6), with 16-byte chunks XOR-swizzled by row so
-// that the fragment reads of 16 rows hit distinct banks.
-constexpr int kGemmThreads = 512;
-constexpr int BM = 128, BN = 64, BK = 64;
+// Block tile: 8 waves in a WGM x WGN grid, each owning WM x WN outputs
+// (real rows x real columns): 128 x 128 with 64 x 32 wave tiles for S <= 5,
+// 128 x 64 with 32 x 32 wave tiles above.  Operand panels (S slices x rows x
+// 64 bytes of depth) go from global memory straight into LDS
+// (global_load_lds_dwordx4: each wave instruction fills 1 KiB = 16 rows x 64
+// bytes, the lanes choosing source addresses so that the 16-byte chunks land
+// XOR-swizzled by row and the fragment reads hit distinct banks).
+// Two LDS stages when they fit: the next depth step streams in while the
+// current one feeds the matrix cores.
+constexpr int BK = 64;
 constexpr int kGroupM = 4;  // tile rows per group of the tile order
+constexpr int kLdsBytes = 160 * 1024;
 
-__device__ __forceinline__ int lds_at(int p, int r, int c, int rows) { return ((p * rows + r) << 2) + (c ^ ((r >> 2) & 3)); }
+// LDS slot (16-byte units) of logical chunk c of row r of slice p.  Fragment
+// reads take 16 rows at one logical chunk; LDS serves them 8 lanes (128
+// bytes over 32 banks) at a time, so rows r and r + 2, r + 4, r + 6 must use
+// different physical chunks: physical = logical ^ ((r / 2) % 4).
+__device__ __forceinline__ int lds_at(int p, int r, int c, int rows) { return ((p * rows + r) << 2) + (c ^ ((r >> 1) & 3)); }
+
+template <int S, int WM, int WN, int WGM, int WGN>
+struct OzShape {
+  static constexpr int BM = WM * WGM, BN = WN * WGN;
+  static constexpr int FA = WM / 16, FB = WN / 16;
+  static constexpr int NST = (2 * S * (BM + BN) * BK <= kLdsBytes) ? 2 : 1;
+  static constexpr int CA = S * BM / 16, CB = S * BN / 16;  // 1 KiB chunks per stage
+  static constexpr int NW = WGM * WGN, THREADS = 64 * NW;
+};
 
 // C (M x N2 real, leading dim ldc in doubles) (+)= recombined slice products.
 // As: S planes of M x K2 int8 (row-major, ld lda); Bt: S planes of N2 x K2 int8.
-template <int S>
-__global__ void __launch_bounds__(kGemmThreads) ozaki_gemm_kernel(
+template <int S, int WM, int WN, int WGM, int WGN>
+__global__ void __launch_bounds__(64 * WGM * WGN) ozaki_gemm_kernel(
     const int8_t *__restrict__ As, uint64_t a_plane, uint64_t lda, const int8_t *__restrict__ Bt, uint64_t b_plane,
     uint64_t ldb, const int *__restrict__ ea, const int *__restrict__ eb, double *__restrict__ C, uint64_t ldc,
     uint64_t M, uint64_t N2, uint64_t K2, unsigned tilesM, unsigned tilesN, int accumulate) {
+  using Z = OzShape<S, WM, WN, WGM, WGN>;
+  constexpr int BM = Z::BM, BN = Z::BN, NST = Z::NST;
+  __shared__ int4_t sA[NST][S * BM * 4];
+  __shared__ int4_t sB[NST][S * BN * 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave / WGN) * WM, wn = (wave % WGN) * WN;
+
+  // Workgroups are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+  // range of tiles, walked in groups of kGroupM tile rows, so the blocks an XCD
+  // runs together share A and B panels in its L2.
+  const unsigned T = gridDim.x, b = blockIdx.x;
+  const unsigned xcd = b & 7, per = T >> 3, rem = T & 7;
+  const unsigned t = xcd * per + min(xcd, rem) + (b >> 3);
+  const unsigned gsize = kGroupM * tilesN;
+  const unsigned first_m = (t / gsize) * kGroupM;
+  const unsigned gm_cnt = min((unsigned)kGroupM, tilesM - first_m);
+  const unsigned in = t % gsize;
+  const uint64_t m0 = (uint64_t)(first_m + in % gm_cnt) * BM;
+  const uint64_t n0 = (uint64_t)(in / gm_cnt) * BN;
+
+  int4_t acc[S][Z::FA][Z::FB];
+#pragma unroll
+  for (int d = 0; d < S; ++d)
+#pragma unroll
+    for (int i = 0; i < Z::FA; ++i)
+#pragma unroll
+      for (int j = 0; j < Z::FB; ++j) acc[d][i][j] = int4_t{0, 0, 0, 0};
+
+  // Lane i of a chunk fills LDS slot i of it: row i / 4 of the chunk's 16,
+  // physical 16-byte column i % 4, which holds logical column (i % 4) ^ (row / 2 % 4).
+  const int crow = lane >> 2, ccol = (lane & 3) ^ ((lane >> 3) & 3);
+  auto issue = [&](int st, uint64_t k0) {
+    for (int c = wave; c < Z::CA + Z::CB; c += Z::NW) {
+      const bool isA = c < Z::CA;
+      const int cc = isA ? c : c - Z::CA;
+      const int rows = isA ? BM : BN;
+      const int p = cc / (rows / 16), g = cc % (rows / 16);
+      const int r = g * 16 + crow;
+      const uint64_t grow = isA ? min(m0 + r, M - 1) : min(n0 + r, N2 - 1);  // rows past the end: any valid row
+      const int8_t *src = isA ? As + p * a_plane + grow * lda : Bt + p * b_plane + grow * ldb;
+      int4_t *dst = isA ? &sA[st][(p * rows + g * 16) * 4] : &sB[st][(p * rows + g * 16) * 4];
+      __builtin_amdgcn_global_load_lds((const void *)(src + k0 + ccol * 16),
+                                       (void __attribute__((address_space(3))) *)dst, 16, 0, 0);
+    }
+  };
+
+  // The caller pads the real depth K2 to a multiple of BK with zeros.
+  const uint64_t nk = K2 / BK;
+  issue(0, 0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  for (uint64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (NST == 2) ? (int)(kt & 1) : 0;
+    if (NST == 2 && kt + 1 < nk) issue(cur ^ 1, (kt + 1) * BK);  // streams in during the MFMAs below
+    // Fragments (16x16x64 i8): lane l holds row / column l % 16, k bytes 16 (l / 16) .. +15.
+    // All B slices stay in registers; A slices stream through, one at a time.
+    int4_t fb[S][Z::FB];
+#pragma unroll
+    for (int q = 0; q < S; ++q)
+#pragma unroll
+      for (int j = 0; j < Z::FB; ++j) fb[q][j] = sB[cur][lds_at(q, wn + 16 * j + (lane & 15), lane >> 4, BN)];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+      int4_t fa[Z::FA];
+#pragma unroll
+      for (int i = 0; i < Z::FA; ++i) fa[i] = sA[cur][lds_at(p, wm + 16 * i + (lane & 15), lane >> 4, BM)];
+#pragma unroll
+      for (int q = 0; q < S - p; ++q)
+#pragma unroll
+        for (int i = 0; i < Z::FA; ++i)
+#pragma unroll
+          for (int j = 0; j < Z::FB; ++j)
+            acc[p + q][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[q][j], acc[p + q][i][j], 0, 0, 0);
+    }
+    if (NST == 1 && kt + 1 < nk) {
+      __syncthreads();  // everyone is done reading the only stage
+      issue(0, (kt + 1) * BK);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+  // 16x16 i32 result layout: lane l holds column l % 16 of rows 4 (l / 16) + v.
+#pragma unroll
+  for (int i = 0; i < Z::FA; ++i)
+#pragma unroll
+    for (int j = 0; j < Z::FB; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint64_t gm = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+        const uint64_t gn = n0 + wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N2) {
+          double sum = 0.0;
+#pragma unroll
+          for (int d = S - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -(12 + 8 * d));
+          sum = ldexp(sum, ea[gm] + eb[gn]);
+          double *c = C + gm * ldc + gn;
+          *c = accumulate ? *c + sum : sum;
+        }
+      }
+}
+
+// Register-staged variant for S >= 6 (128 x 64 tiles, 8 waves of 32 x 32):
+// each thread loads its 16-byte chunks of the next depth step into registers
+// before the MFMAs of the current one and writes them to the other LDS stage
+// after them.  Same arithmetic and result layout as ozaki_gemm_kernel.
+template <int S>
+__global__ void __launch_bounds__(512) ozaki_gemm_staged(
+    const int8_t *__restrict__ As, uint64_t a_plane, uint64_t lda, const int8_t *__restrict__ Bt, uint64_t b_plane,
+    uint64_t ldb, const int *__restrict__ ea, const int *__restrict__ eb, double *__restrict__ C, uint64_t ldc,
+    uint64_t M, uint64_t N2, uint64_t K2, unsigned tilesM, unsigned tilesN, int accumulate) {
+  constexpr int BM = 128, BN = 64, kGemmThreads = 512;
   constexpr int NST = (S <= 6) ? 2 : 1;
   __shared__ int4_t sA[NST][S * BM * 4];
   __shared__ int4_t sB[NST][S * BN * 4];
@@ -209,7 +348,7 @@ __global__ void __launch_bounds__(kGemmThreads) ozaki_gemm_kernel(
         if (gm < M && gn < N2) {
           double sum = 0.0;
 #pragma unroll
-          for (int d = S - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -(12 + 7 * d));
+          for (int d = S - 1; d >= 0; --d) sum += ldexp((double)acc[d][i][j][v], -(12 + 8 * d));
           sum = ldexp(sum, ea[gm] + eb[gn]);
           double *c = C + gm * ldc + gn;
           *c = accumulate ? *c + sum : sum;
@@ -217,28 +356,44 @@ __global__ void __launch_bounds__(kGemmThreads) ozaki_gemm_kernel(
       }
 }
 
+
+// Wide tiles while the stages fit two-deep; narrower ones for more slices.
 template <int S>
-void launch_ozaki(hipStream_t st, unsigned tilesM, unsigned tilesN, const int8_t *As, uint64_t ap, uint64_t lda,
-                  const int8_t *Bt, uint64_t bp, uint64_t ldb, const int *ea, const int *eb, double *C, uint64_t ldc,
-                  uint64_t M, uint64_t N2, uint64_t K2, int accumulate) {
-  hipLaunchKernelGGL((ozaki_gemm_kernel<S>), dim3(tilesM * tilesN), dim3(kGemmThreads), 0, st, As, ap, lda, Bt, bp, ldb,
-                     ea, eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
+struct OzPick {
+  static constexpr int WM = 64, WN = 32, WGM = 2, WGN = 4;
+  using Z = OzShape<S, WM, WN, WGM, WGN>;
+};
+
+template <int S>
+void launch_ozaki(hipStream_t st, const int8_t *As, uint64_t ap, uint64_t lda, const int8_t *Bt, uint64_t bp,
+                  uint64_t ldb, const int *ea, const int *eb, double *C, uint64_t ldc, uint64_t M, uint64_t N2,
+                  uint64_t K2, int accumulate) {
+  if constexpr (S >= 6) {
+    const unsigned tilesM = (unsigned)((M + 127) / 128), tilesN = (unsigned)((N2 + 63) / 64);
+    hipLaunchKernelGGL((ozaki_gemm_staged<S>), dim3(tilesM * tilesN), dim3(512), 0, st, As, ap, lda, Bt, bp, ldb, ea,
+                       eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
+  } else {
+  using P = OzPick<S>;
+  const unsigned tilesM = (unsigned)((M + P::Z::BM - 1) / P::Z::BM), tilesN = (unsigned)((N2 + P::Z::BN - 1) / P::Z::BN);
+  hipLaunchKernelGGL((ozaki_gemm_kernel<S, P::WM, P::WN, P::WGM, P::WGN>), dim3(tilesM * tilesN), dim3(P::Z::THREADS),
+                     0, st, As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
+  }
 }
 
 }  // namespace
 
-// Number of 7-bit slices of the split-precision GEMM (default 6).
-static int g_split_slices = 6;
+// Number of 8-bit slices of the split-precision GEMM (default 5).
+static int g_split_slices = 5;
 
 extern "C" void qtnh_tune_split(int slices) { g_split_slices = std::max(1, std::min(slices, kMaxSlices)); }
 
 int split_slices() { return g_split_slices; }
 
-// A-priori componentwise bound of the truncated scheme: every slice residual is
-// at most 2^-7S of its row scale and the dropped diagonals d >= S add at most
-// (S + 1) 2^-7S per real product, so with 2^e <= 2 max|x| for both scales
-//   |C_ij - C~_ij| <= eta max_k |A_ik| max_k |B_kj|,  eta = 4 (S + 3) 2^-7S (2K).
-double split_error_bound(int S, uint64_t K) { return 4.0 * (double)(S + 3) * ldexp(1.0, -7 * S) * (double)(2 * K); }
+// A-priori componentwise bound of the truncated scheme: each scaled value is
+// within 2^-(8S-1) of its digits, the dropped diagonals d >= S add at most
+// (S + 1) 2^(2-8S) per real product, so with 2^e <= 2 max|x| for both scales
+//   |C_ij - C~_ij| <= eta max_k |A_ik| max_k |B_kj|,  eta = 16 (S + 2) 2^-8S (2K).
+double split_error_bound(int S, uint64_t K) { return 16.0 * (double)(S + 2) * ldexp(1.0, -8 * S) * (double)(2 * K); }
 
 qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
                                uint64_t K, void *scratch, uint64_t scratch_bytes, double *err_out) {
@@ -266,18 +421,19 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
                      b_plane, K2, eb);
   st = check_hip(ctx, hipGetLastError(), "split B launch");
   if (st) return st;
-  // int32 accumulators stay exact while (S pairs) * 64 * 64 * depth < 2^31: run depth chunks of 2^15.
-  const uint64_t chunk = 32768;
-  const uint64_t tilesM = (M + BM - 1) / BM, tilesN = (N2 + BN - 1) / BN;
-  if (tilesM * tilesN > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "split: grid too large");
+  // int32 accumulators stay exact while (S pairs) * 128 * 128 * depth < 2^31:
+  // depth chunks of 2^13 (S <= 8).
+  const uint64_t chunk = 8192;
+  if (((M + 127) / 128) * ((N2 + 63) / 64) > 0x7fffffffull)
+    return set_error(ctx, QTNH_EUNSUPPORTED, "split: grid too large");
   for (uint64_t k0 = 0; k0 < K2; k0 += chunk) {
     const uint64_t kc = std::min(chunk, K2 - k0);
     const int accumulate = k0 > 0;
     switch (S) {
 #define QTNH_SPLIT_CASE(s)                                                                                            \
   case s:                                                                                                            \
-    launch_ozaki<s>(ctx->stream, (unsigned)tilesM, (unsigned)tilesN, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, \
-                    (double *)C, N2, M, N2, kc, accumulate);                                                         \
+    launch_ozaki<s>(ctx->stream, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, (double *)C, N2, M, N2, kc,     \
+                    accumulate);                                                                                     \
     break;
       QTNH_SPLIT_CASE(1)
       QTNH_SPLIT_CASE(2)
@@ -287,8 +443,8 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
       QTNH_SPLIT_CASE(6)
       QTNH_SPLIT_CASE(7)
       default:
-        launch_ozaki<8>(ctx->stream, (unsigned)tilesM, (unsigned)tilesN, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea,
-                        eb, (double *)C, N2, M, N2, kc, accumulate);
+        launch_ozaki<8>(ctx->stream, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, (double *)C, N2, M, N2, kc,
+                        accumulate);
         break;
 #undef QTNH_SPLIT_CASE
     }

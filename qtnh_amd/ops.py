@@ -223,8 +223,8 @@ def tune_zgemm(bn=0):
     lib().qtnh_tune_zgemm(bn)
 
 
-def tune_split(slices=6):
-    """Number of 7-bit slices of split-precision contractions (process-wide)."""
+def tune_split(slices=5):
+    """Number of 8-bit slices of split-precision contractions (process-wide)."""
     lib().qtnh_tune_split(slices)
 
 

@@ -2,10 +2,10 @@
 
 Checked against the oracle (small sizes) and against FP64 complex GEMM
 (torch.matmul on complex128, larger sizes):
-  * normwise relative error <= 1e-10 at the default 6 slices;
+  * normwise relative error <= 1e-10 at the default 5 slices;
   * the componentwise a-priori bound returned in err_out holds,
     |C - C~|_ij <= eta max_k|A_ik| max_k|B_kj| (plus FP64 rounding slack);
-  * the error falls by about 2^-7 per extra slice;
+  * the error falls by about 2^-8 per extra slice;
   * rows and columns of very different magnitude, zero rows, ragged
     (non-power-of-two) dims and placements all keep the bound;
   * problems small enough for the direct route, and few-output long sums,
@@ -29,7 +29,7 @@ def _gpu():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     yield
-    ops.tune_split(6)
+    ops.tune_split(5)
 
 
 def dev(a):
@@ -63,7 +63,7 @@ def check_bound(A, B, C, Cref, eta, K):
     assert excess.max() <= 0, excess.max()
 
 
-def split_gemm(A, B, slices=6):
+def split_gemm(A, B, slices=5):
     """C = A @ B through qtnh_contract_c128 with power-of-two dims (M, K, N powers of 2)."""
     M, K = A.shape
     N = B.shape[1]
@@ -116,8 +116,8 @@ def test_split_error_falls_with_slices():
         errs.append(np.linalg.norm(C - ref) / np.linalg.norm(ref))
         check_bound(A, B, C, ref, eta, 512)
     for s in range(1, 6):
-        # each slice adds 7 bits; allow slack for the random constant
-        assert errs[s] < errs[s - 1] * 2.0 ** -4, errs
+        # each slice adds 8 bits; allow slack for the random constant
+        assert errs[s] < errs[s - 1] * 2.0 ** -5, errs
     assert errs[7] < 1e-14, errs
 
 
@@ -134,9 +134,10 @@ def test_split_wide_dynamic_range_and_zero_rows():
     ref = A @ B
     assert np.all(C[7] == 0) and np.all(C[:, 3] == 0)
     check_bound(A, B, C, ref, eta, 256)
-    # row-relative error: each row has its own scale
+    # row-relative error: each row has its own scale (rows whose result
+    # cancels below max|A_i| max|B_j| can lose more; the bound above holds)
     rel = np.abs(C - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-300)
-    assert np.all(rel[np.abs(ref).max(axis=1) > 0] <= 1e-9), rel.max()
+    assert np.all(rel[np.abs(ref).max(axis=1) > 0] <= 1e-8), rel.max()
 
 
 @pytest.mark.parametrize("dA,dB,wires", [
