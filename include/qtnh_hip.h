@@ -208,6 +208,19 @@ qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t o
  * qtnh_net_run.  *waves (may be NULL) gets the number of waves.  Blocks. */
 qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, double *const *outs,
                                uint64_t budget_bytes, int *waves);
+/* A prepared batch for repeated evaluation (same structures, new leaf values):
+ * device memory, descriptors and the launch sequence are built once; with
+ * use_graph the sequence is captured into a HIP graph (on a private stream)
+ * and each run is one graph launch on the context's stream.  The networks
+ * must outlive the batch and must not be re-planned. */
+typedef struct qtnh_net_batch_s *qtnh_net_batch_t;
+qtnh_status qtnh_net_batch_create(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, int use_graph,
+                                  qtnh_net_batch_t *out);
+/* New values (host, same size) for leaf `leaf` of network `net`; stream-ordered. */
+qtnh_status qtnh_net_batch_set_leaf(qtnh_ctx_t ctx, qtnh_net_batch_t b, int net, int leaf, const double *data);
+/* Contracts every network of the batch; outs[i] (host) receives network i's result.  Blocks. */
+qtnh_status qtnh_net_batch_run(qtnh_ctx_t ctx, qtnh_net_batch_t b, double *const *outs);
+qtnh_status qtnh_net_batch_destroy(qtnh_net_batch_t b);
 
 /* ---- serving loop (no reference counterpart): queries on the output states of
  * seeded random SU(2) + CZ circuits (qtnh_circuit_random), answered on the
@@ -236,6 +249,12 @@ typedef struct {
  * Answers are identical under both policies. */
 qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *queries, qtnh_query_result *results,
                        uint64_t budget_bytes, int policy);
+
+/* Times the fused executor at several tile shapes on a depth-2 random
+ * circuit of min(n, 26) qubits and keeps the fastest for n-qubit circuits
+ * run on this context (results do not depend on it).  Outputs may be NULL:
+ * the chosen tile bits, low bits and the best time (ms). */
+qtnh_status qtnh_autotune_circuit(qtnh_ctx_t ctx, int n_qubits, int *tile_bits, int *low_bits, double *ms);
 
 /* ---- queries on a state (no reference counterpart) ---- */
 /* out[j] = <psi| Z_{qubits[j]} |psi> = sum_x |psi_x|^2 (1 - 2 x_q), fixed

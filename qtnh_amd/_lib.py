@@ -85,6 +85,11 @@ SIGNATURES = {
     "qtnh_net_run": [_p, _p, _dp, _u64],
     "qtnh_net_run_batch": [_p, _i, ctypes.POINTER(_p), ctypes.POINTER(_dp), _u64, ctypes.POINTER(_i)],
     "qtnh_serve": [_p, _i, _p, _p, _u64, _i],
+    "qtnh_net_batch_create": [_p, _i, ctypes.POINTER(_p), _i, ctypes.POINTER(_p)],
+    "qtnh_net_batch_set_leaf": [_p, _p, _i, _i, _dp],
+    "qtnh_net_batch_run": [_p, _p, ctypes.POINTER(_dp)],
+    "qtnh_net_batch_destroy": [_p],
+    "qtnh_autotune_circuit": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _dp],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_zgemm": None,

@@ -371,10 +371,10 @@ uint64_t mask_of(const qtnh_circuit_s &c, const qtnh_circuit_s::Op &op) {
 
 // Greedy partition of the op sequence into runs of at most tb distinct bits,
 // each widened with the lowest free bits (coalesced tile rows) up to tb.
-void plan_runs(qtnh_circuit_s &c, int tb) {
+void plan_runs(qtnh_circuit_s &c, int tb, int low_bits) {
   c.runs.clear();
   const int tbe = std::min(tb, c.n);
-  const uint64_t low = (1ull << std::min(std::min(g_low_bits, tbe - 3 > 0 ? tbe - 3 : 0), c.n)) - 1;
+  const uint64_t low = (1ull << std::min(std::min(low_bits, tbe - 3 > 0 ? tbe - 3 : 0), c.n)) - 1;
   uint64_t cur = low;
   uint32_t begin = 0;
   auto close = [&](uint32_t end) {
@@ -408,15 +408,23 @@ void free_compiled(qtnh_circuit_s &c) {
   c.compiled_tb = -1;
 }
 
+// Tile shape for an n-qubit circuit: the autotuned one of this context, else the process default.
+void tile_shape(qtnh_ctx_t ctx, int n, int *tb, int *low) {
+  const bool tuned = n < 64 && ctx->circ_tb[n] > 0;
+  *tb = tuned ? ctx->circ_tb[n] : g_tile_bits;
+  *low = tuned ? ctx->circ_low[n] : g_low_bits;
+}
+
 qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
-  if (c.compiled_tb == g_tile_bits * 16 + g_low_bits && c.device == ctx->device && c.compiled_ops == c.ops.size())
-    return QTNH_OK;
+  int tb, low;
+  tile_shape(ctx, c.n, &tb, &low);
+  if (c.compiled_tb == tb * 16 + low && c.device == ctx->device && c.compiled_ops == c.ops.size()) return QTNH_OK;
   if (c.d_ops || c.d_coef) {
     qtnh_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "circuit recompile sync");
     if (st) return st;
     free_compiled(c);
   }
-  plan_runs(c, g_tile_bits);
+  plan_runs(c, tb, low);
   std::vector<TileOp> tops(std::max<size_t>(c.ops.size(), 1));
   std::vector<double> coef;
   for (const auto &r : c.runs) {
@@ -453,7 +461,7 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
     return st;
   }
   c.tops = tops;
-  c.compiled_tb = g_tile_bits * 16 + g_low_bits;
+  c.compiled_tb = tb * 16 + low;
   c.device = ctx->device;
   c.compiled_ops = c.ops.size();
   return QTNH_OK;
@@ -784,7 +792,7 @@ qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, 
     qtnh_circuit_s tmp;
     tmp.n = c->n;
     tmp.ops = c->ops;
-    plan_runs(tmp, g_tile_bits);
+    plan_runs(tmp, g_tile_bits, g_low_bits);
     *n_runs = tmp.runs.size();
   }
   return QTNH_OK;
@@ -838,6 +846,65 @@ qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit
   }
   starts.push_back(descs.size());
   return launch_run_groups(ctx, descs, starts);
+}
+
+qtnh_status qtnh_autotune_circuit(qtnh_ctx_t ctx, int n_qubits, int *tile_bits, int *low_bits, double *ms_out) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (n_qubits < 1 || n_qubits > 62) return set_error(ctx, QTNH_EINVAL, "autotune: bad qubit count");
+  // Proxy: a depth-2 random circuit on min(n, 26) qubits (the best tile does
+  // not depend on the state size once it spans many tiles).
+  const int np = std::min(n_qubits, 26);
+  const uint64_t N = 1ull << np;
+  void *w = nullptr;
+  st = workspace(ctx, N * sizeof(c128), &w);
+  if (st) return st;
+  st = launch_fill(ctx, (c128 *)w, N, 1, QTNH_DIST_NORMAL);
+  if (st) return st;
+  const int cand[][2] = {{10, 3}, {11, 3}, {11, 4}, {12, 4}, {12, 5}, {13, 4}, {13, 5}};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  st = check_hip(ctx, hipEventCreate(&e0), "autotune event");
+  if (!st) st = check_hip(ctx, hipEventCreate(&e1), "autotune event");
+  double best = 1e30;
+  int btb = g_tile_bits, blow = g_low_bits;
+  for (const auto &cd : cand) {
+    if (st) break;
+    if (cd[0] > std::max(np, 10)) continue;
+    qtnh_circuit_s c;
+    c.n = np;
+    (void)qtnh_circuit_random(&c, 2, 12345);
+    ctx->circ_tb[np] = (int8_t)cd[0];
+    ctx->circ_low[np] = (int8_t)cd[1];
+    st = run_fused(ctx, c, (c128 *)w);  // compile + warm up
+    if (!st) st = check_hip(ctx, hipEventRecord(e0, ctx->stream), "autotune record");
+    if (!st) st = run_fused(ctx, c, (c128 *)w);
+    if (!st) st = check_hip(ctx, hipEventRecord(e1, ctx->stream), "autotune record");
+    if (!st) st = check_hip(ctx, hipEventSynchronize(e1), "autotune sync");
+    float ms = 0;
+    if (!st) st = check_hip(ctx, hipEventElapsedTime(&ms, e0, e1), "autotune elapsed");
+    if (!st && ms < best) {
+      best = ms;
+      btb = cd[0];
+      blow = cd[1];
+    }
+    free_compiled(c);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) {
+    ctx->circ_tb[np] = 0;
+    return st;
+  }
+  ctx->circ_tb[n_qubits] = (int8_t)btb;
+  ctx->circ_low[n_qubits] = (int8_t)blow;
+  if (np != n_qubits) {
+    ctx->circ_tb[np] = (int8_t)btb;
+    ctx->circ_low[np] = (int8_t)blow;
+  }
+  if (tile_bits) *tile_bits = btb;
+  if (low_bits) *low_bits = blow;
+  if (ms_out) *ms_out = best;
+  return QTNH_OK;
 }
 
 qtnh_status qtnh_expect_z_c128(qtnh_ctx_t ctx, const void *state, int n_qubits, int nq, const uint16_t *qubits,

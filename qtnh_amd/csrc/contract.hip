@@ -194,7 +194,7 @@ qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const 
   }
   if (P.K == 0) {
     // Empty sums: the result is zero.
-    return check_hip(ctx, hipMemsetAsync(C, 0, P.nC * sizeof(c128), ctx->stream), "contract zero fill");
+    return launch_zero(ctx, C, P.nC * sizeof(c128));
   }
   // Natural axis q -> source stride (A or B); result axis p holds natural axis inv[p].
   const int n1 = (int)P.open1.size();

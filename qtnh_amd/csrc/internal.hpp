@@ -27,6 +27,9 @@ struct qtnh_ctx_s {
   // Growable device buffer for reduction partials (expectation values).
   void *red = nullptr;
   uint64_t red_bytes = 0;
+  // Fused-circuit tile shape chosen by the autotuner, per qubit count (0: unset).
+  int8_t circ_tb[64] = {0};
+  int8_t circ_low[64] = {0};
 };
 
 namespace qtnh_hip {
@@ -95,6 +98,8 @@ qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const 
                                    const uint16_t *repl2, c128 *C);
 qtnh_status launch_norm2(qtnh_ctx_t ctx, const c128 *x, uint64_t n, double *out);
 qtnh_status launch_scale(qtnh_ctx_t ctx, c128 *x, uint64_t n, double re, double im);
+// Zero fill with a kernel (capturable in graphs, unlike a memset node).
+qtnh_status launch_zero(qtnh_ctx_t ctx, void *p, uint64_t bytes);
 
 // Complex GEMM C[MxN] = A[MxK] * B[KxN], row-major with leading dims lda/ldb/ldc (elements).
 qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N,

@@ -406,6 +406,158 @@ uint64_t gdesc_blocks(const GDesc &d) { return (d.nC + kGThreads - 1) / kGThread
 }  // namespace
 }  // namespace qtnh_hip
 
+namespace qtnh_hip {
+namespace {
+
+// One batch of networks placed in one device block:
+//   [leaves of every net | arenas | packed results | GEMM workspace]
+// plus descriptor memory (grouped-step descriptors, result gather table).
+struct Wave {
+  std::vector<const qtnh_net_s *> nets;
+  std::vector<Where> blocks;
+  std::vector<double> leaves;  // host copy of every leaf, packed as on the device
+  std::vector<GDesc> descs;
+  std::vector<size_t> starts;  // descriptors of step index s: [starts[s], starts[s + 1])
+  std::vector<Gather> gathers;
+  c128 *base = nullptr, *res = nullptr;
+  char *gws = nullptr;
+  uint64_t leaf_elems = 0, res_elems = 0, ws_bytes = 0;
+  size_t steps = 0;
+  GDesc *ddesc = nullptr;
+  Gather *dgath = nullptr;
+
+  static uint64_t device_bytes(const qtnh_net_t *nets, int count) {
+    uint64_t elems = 0, res = 0, ws = 0;
+    for (int i = 0; i < count; ++i) {
+      elems += block_elems(*nets[i]);
+      res += nets[i]->nodes[nets[i]->result].size;
+      ws = std::max(ws, nets[i]->ws_bytes);
+    }
+    return (elems + round16(res)) * sizeof(c128) + ws;
+  }
+
+  void layout(qtnh_ctx_t, const qtnh_net_t *ns, int count, char *mem) {
+    nets.assign(ns, ns + count);
+    uint64_t elems = 0;
+    for (auto *n : nets) {
+      elems += block_elems(*n);
+      res_elems += n->nodes[n->result].size;
+      leaf_elems += round16(n->leaf_elems);
+      ws_bytes = std::max(ws_bytes, n->ws_bytes);
+      steps = std::max(steps, n->steps.size());
+    }
+    base = (c128 *)mem;
+    res = base + elems;
+    gws = (char *)(res + round16(res_elems));
+    leaves.assign(2 * std::max<uint64_t>(leaf_elems, 1), 0.0);
+    uint64_t loff = 0, aoff = leaf_elems;
+    for (auto *n : nets) {
+      blocks.push_back({base + loff, base + aoff});
+      memcpy(leaves.data() + 2 * loff, n->leaf_data.data(), n->leaf_elems * sizeof(c128));
+      loff += round16(n->leaf_elems);
+      aoff += round16(n->arena_elems);
+    }
+    starts.assign(steps + 1, 0);
+    for (size_t s = 0; s < steps; ++s) {
+      starts[s] = descs.size();
+      uint64_t first = 0;
+      for (size_t i = 0; i < nets.size(); ++i) {
+        const auto &n = *nets[i];
+        if (s >= n.steps.size() || !n.steps[s].direct || !n.steps[s].binary) continue;
+        GDesc d = make_gdesc(n, blocks[i], s, first);
+        first += gdesc_blocks(d);
+        descs.push_back(d);
+      }
+    }
+    starts[steps] = descs.size();
+    uint64_t dst = 0;
+    for (size_t i = 0; i < nets.size(); ++i) {
+      const auto &n = *nets[i];
+      gathers.push_back({(const double2_t *)node_ptr(n, blocks[i], n.result), n.nodes[n.result].size, dst});
+      dst += n.nodes[n.result].size;
+    }
+  }
+
+  uint64_t desc_bytes() const {
+    return ((descs.size() * sizeof(GDesc) + 255) & ~255ull) + gathers.size() * sizeof(Gather) + 256;
+  }
+
+  void place_descriptors(void *mem) {
+    ddesc = (GDesc *)mem;
+    dgath = (Gather *)((char *)mem + ((descs.size() * sizeof(GDesc) + 255) & ~255ull));
+  }
+
+  // Leaves and descriptors to the device (stream-ordered).
+  qtnh_status upload(qtnh_ctx_t ctx) {
+    qtnh_status st = check_hip(
+        ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
+        "batch leaf upload");
+    if (!st && !descs.empty())
+      st = check_hip(ctx, hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(GDesc), hipMemcpyHostToDevice,
+                                         ctx->stream),
+                     "batch descriptor upload");
+    if (!st)
+      st = check_hip(ctx, hipMemcpyAsync(dgath, gathers.data(), gathers.size() * sizeof(Gather), hipMemcpyHostToDevice,
+                                         ctx->stream),
+                     "batch gather upload");
+    return st;
+  }
+
+  // Every kernel of the batch, in step order, then the result gather.  Only
+  // kernel launches (no copies, fills or allocations): capturable.
+  qtnh_status enqueue(qtnh_ctx_t ctx) {
+    for (size_t s = 0; s < steps; ++s) {
+      const size_t b = starts[s], e = starts[s + 1];
+      if (e > b) {
+        const uint64_t tb = descs[e - 1].first + gdesc_blocks(descs[e - 1]);
+        const unsigned grid = (unsigned)std::min<uint64_t>(tb, (uint64_t)ctx->num_cus * 16);
+        hipLaunchKernelGGL(grouped_direct_kernel, dim3(grid), dim3(kGThreads), 0, ctx->stream, ddesc + b, (int)(e - b),
+                           tb);
+        qtnh_status st = check_hip(ctx, hipGetLastError(), "grouped contraction launch");
+        if (st) return st;
+      }
+      for (size_t i = 0; i < nets.size(); ++i) {
+        const auto &n = *nets[i];
+        if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary)) continue;
+        qtnh_status st = run_step(ctx, n, blocks[i], s, gws, ws_bytes);
+        if (st) return st;
+      }
+    }
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)gathers.size()), dim3(64), 0, ctx->stream, dgath,
+                       (int)gathers.size(), (double2_t *)res);
+    return check_hip(ctx, hipGetLastError(), "gather launch");
+  }
+
+  // Packed results to the callers' host buffers (blocks).
+  qtnh_status fetch(qtnh_ctx_t ctx, double *const *outs) {
+    std::vector<double> hres(2 * std::max<uint64_t>(res_elems, 1));
+    qtnh_status st = check_hip(
+        ctx, hipMemcpyAsync(hres.data(), res, res_elems * sizeof(c128), hipMemcpyDeviceToHost, ctx->stream),
+        "batch result copy");
+    if (!st) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
+    if (st) return st;
+    uint64_t r = 0;
+    for (size_t i = 0; i < nets.size(); ++i) {
+      const uint64_t n = nets[i]->nodes[nets[i]->result].size;
+      memcpy(outs[i], hres.data() + 2 * r, n * sizeof(c128));
+      r += n;
+    }
+    return QTNH_OK;
+  }
+};
+
+}  // namespace
+}  // namespace qtnh_hip
+
+struct qtnh_net_batch_s {
+  int device = 0;
+  std::vector<qtnh_net_t> nets;
+  qtnh_hip::Wave wave;
+  void *mem = nullptr, *desc = nullptr;
+  hipStream_t capture = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 using namespace qtnh_hip;
 
 extern "C" {
@@ -578,119 +730,117 @@ qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets
   // (the GEMM workspace is shared by the wave, whose large steps run in turn).
   int waves = 0;
   for (int w0 = 0; w0 < count;) {
-    uint64_t elems = 0, wsb = 0;
+    uint64_t bytes = 0;
     int w1 = w0;
     while (w1 < count) {
-      const uint64_t e2 = elems + block_elems(*nets[w1]);
-      const uint64_t ws2 = std::max(wsb, nets[w1]->ws_bytes);
-      if (w1 > w0 && e2 * sizeof(c128) + ws2 > budget_bytes) break;
-      elems = e2;
-      wsb = ws2;
+      const uint64_t b2 = Wave::device_bytes(nets + w0, w1 + 1 - w0);
+      if (w1 > w0 && b2 > budget_bytes) break;
+      bytes = b2;
       ++w1;
     }
     ++waves;
-    // Layout: [leaves of every net | arenas | results | gemm workspace]; the
-    // leaves go up in one copy, the arenas are written before they are read.
-    uint64_t res_elems = 0, leaf_elems = 0;
-    for (int i = w0; i < w1; ++i) {
-      res_elems += nets[i]->nodes[nets[i]->result].size;
-      leaf_elems += round16(nets[i]->leaf_elems);
-    }
-    const uint64_t total = (elems + round16(res_elems)) * sizeof(c128) + wsb;
     void *wp = nullptr;
-    st = workspace(ctx, total, &wp);
+    st = workspace(ctx, bytes, &wp);
     if (st) return st;
-    c128 *base = (c128 *)wp;
-    c128 *res = base + elems;
-    char *gws = (char *)(res + round16(res_elems));
-    std::vector<Where> blocks(w1 - w0);
-    std::vector<double> leaves(2 * std::max<uint64_t>(leaf_elems, 1), 0.0);
-    uint64_t loff = 0, aoff = leaf_elems;
-    size_t steps = 0;
-    for (int i = w0; i < w1; ++i) {
-      blocks[i - w0] = {base + loff, base + aoff};
-      memcpy(leaves.data() + 2 * loff, nets[i]->leaf_data.data(), nets[i]->leaf_elems * sizeof(c128));
-      loff += round16(nets[i]->leaf_elems);
-      aoff += round16(nets[i]->arena_elems);
-      steps = std::max(steps, nets[i]->steps.size());
-    }
-    st = check_hip(ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice,
-                                       ctx->stream),
-                   "batch leaf upload");
+    Wave w;
+    w.layout(ctx, nets + w0, w1 - w0, (char *)wp);
+    st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, w.desc_bytes());
     if (st) return st;
-    // Grouped descriptors for every step index, uploaded once.
-    std::vector<GDesc> descs;
-    std::vector<size_t> starts(steps + 1, 0);
-    for (size_t s = 0; s < steps; ++s) {
-      starts[s] = descs.size();
-      uint64_t first = 0;
-      for (int i = w0; i < w1; ++i) {
-        const auto &n = *nets[i];
-        if (s >= n.steps.size() || !n.steps[s].direct || !n.steps[s].binary) continue;
-        GDesc d = make_gdesc(n, blocks[i - w0], s, first);
-        first += gdesc_blocks(d);
-        descs.push_back(d);
-      }
-    }
-    starts[steps] = descs.size();
-    std::vector<Gather> gathers;
-    uint64_t dst = 0;
-    for (int i = w0; i < w1; ++i) {
-      const auto &n = *nets[i];
-      gathers.push_back({(const double2_t *)node_ptr(n, blocks[i - w0], n.result), n.nodes[n.result].size, dst});
-      dst += n.nodes[n.result].size;
-    }
-    const uint64_t dbytes = descs.size() * sizeof(GDesc) + gathers.size() * sizeof(Gather) + 256;
-    st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, dbytes);
+    w.place_descriptors(ctx->desc);
+    st = w.upload(ctx);
+    if (!st) st = w.enqueue(ctx);
+    if (!st) st = w.fetch(ctx, outs + w0);
     if (st) return st;
-    GDesc *ddesc = (GDesc *)ctx->desc;
-    Gather *dgath = (Gather *)((char *)ctx->desc + ((descs.size() * sizeof(GDesc) + 255) & ~255ull));
-    if (!descs.empty()) {
-      st = check_hip(ctx, hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(GDesc), hipMemcpyHostToDevice,
-                                         ctx->stream),
-                     "batch descriptor upload");
-      if (st) return st;
-    }
-    st = check_hip(ctx, hipMemcpyAsync(dgath, gathers.data(), gathers.size() * sizeof(Gather), hipMemcpyHostToDevice,
-                                       ctx->stream),
-                   "batch gather upload");
-    if (st) return st;
-    for (size_t s = 0; s < steps; ++s) {
-      const size_t b = starts[s], e = starts[s + 1];
-      if (e > b) {
-        const uint64_t tb = descs[e - 1].first + gdesc_blocks(descs[e - 1]);
-        const unsigned grid = (unsigned)std::min<uint64_t>(tb, (uint64_t)ctx->num_cus * 16);
-        hipLaunchKernelGGL(grouped_direct_kernel, dim3(grid), dim3(kGThreads), 0, ctx->stream, ddesc + b, (int)(e - b),
-                           tb);
-        st = check_hip(ctx, hipGetLastError(), "grouped contraction launch");
-        if (st) return st;
-      }
-      for (int i = w0; i < w1; ++i) {
-        const auto &n = *nets[i];
-        if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary)) continue;
-        st = run_step(ctx, n, blocks[i - w0], s, gws, wsb);
-        if (st) return st;
-      }
-    }
-    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)gathers.size()), dim3(64), 0, ctx->stream, dgath,
-                       (int)gathers.size(), (double2_t *)res);
-    st = check_hip(ctx, hipGetLastError(), "gather launch");
-    if (st) return st;
-    std::vector<double> hres(2 * std::max<uint64_t>(res_elems, 1));
-    st = check_hip(ctx, hipMemcpyAsync(hres.data(), res, res_elems * sizeof(c128), hipMemcpyDeviceToHost, ctx->stream),
-                   "batch result copy");
-    if (st) return st;
-    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "batch sync");
-    if (st) return st;
-    uint64_t r = 0;
-    for (int i = w0; i < w1; ++i) {
-      const uint64_t n = nets[i]->nodes[nets[i]->result].size;
-      memcpy(outs[i], hres.data() + 2 * r, n * sizeof(c128));
-      r += n;
-    }
     w0 = w1;
   }
   if (waves_out) *waves_out = waves;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_batch_create(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, int use_graph,
+                                  qtnh_net_batch_t *out) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!out || count < 1 || !nets) return set_error(ctx, QTNH_EINVAL, "net_batch_create: bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < count; ++i)
+    if (!nets[i] || !nets[i]->planned)
+      return set_error(ctx, QTNH_EINVAL, "net_batch_create: network %d is null or has no plan", i);
+  auto *b = new (std::nothrow) qtnh_net_batch_s;
+  if (!b) return set_error(ctx, QTNH_ENOMEM, "net_batch_create: host allocation");
+  b->device = ctx->device;
+  b->nets.assign(nets, nets + count);
+  auto fail = [&](qtnh_status s) {
+    qtnh_net_batch_destroy(b);
+    return s;
+  };
+  // Buffers of its own: a captured graph holds their addresses.
+  st = check_hip(ctx, hipMalloc(&b->mem, Wave::device_bytes(nets, count)), "net_batch_create alloc");
+  if (st) return fail(st);
+  b->wave.layout(ctx, nets, count, (char *)b->mem);
+  st = check_hip(ctx, hipMalloc(&b->desc, b->wave.desc_bytes()), "net_batch_create alloc");
+  if (st) return fail(st);
+  b->wave.place_descriptors(b->desc);
+  st = b->wave.upload(ctx);
+  if (st) return fail(st);
+  if (use_graph) {
+    // Capture on a private stream (the context's may be the legacy default
+    // stream); replays are launched on the context's stream.
+    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "net_batch_create sync");
+    if (!st) st = check_hip(ctx, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "capture stream");
+    if (st) return fail(st);
+    hipStream_t saved = ctx->stream;
+    ctx->stream = b->capture;
+    st = check_hip(ctx, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (!st) {
+      qtnh_status st2 = b->wave.enqueue(ctx);
+      hipGraph_t g = nullptr;
+      qtnh_status st3 = check_hip(ctx, hipStreamEndCapture(b->capture, &g), "end capture");
+      st = st2 ? st2 : st3;
+      if (!st) st = check_hip(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+      if (g) (void)hipGraphDestroy(g);
+    }
+    ctx->stream = saved;
+    if (st) return fail(st);
+  }
+  *out = b;
+  return QTNH_OK;
+}
+
+qtnh_status qtnh_net_batch_set_leaf(qtnh_ctx_t ctx, qtnh_net_batch_t b, int net, int leaf, const double *data) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!b || !data || net < 0 || net >= (int)b->nets.size())
+    return set_error(ctx, QTNH_EINVAL, "net_batch_set_leaf: bad arguments");
+  const qtnh_net_s &n = *b->nets[net];
+  if (leaf < 0 || (size_t)leaf >= n.n_leaves) return set_error(ctx, QTNH_EINVAL, "net_batch_set_leaf: no leaf %d", leaf);
+  c128 *dst = node_ptr(n, b->wave.blocks[net], leaf);
+  return check_hip(ctx, hipMemcpyAsync(dst, data, n.nodes[leaf].size * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
+                   "net_batch_set_leaf copy");
+}
+
+qtnh_status qtnh_net_batch_run(qtnh_ctx_t ctx, qtnh_net_batch_t b, double *const *outs) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!b || !outs) return set_error(ctx, QTNH_EINVAL, "net_batch_run: bad arguments");
+  if (b->device != ctx->device) return set_error(ctx, QTNH_EINVAL, "net_batch_run: batch of another device");
+  if (b->exec)
+    st = check_hip(ctx, hipGraphLaunch(b->exec, ctx->stream), "graph launch");
+  else
+    st = b->wave.enqueue(ctx);
+  if (st) return st;
+  return b->wave.fetch(ctx, outs);
+}
+
+qtnh_status qtnh_net_batch_destroy(qtnh_net_batch_t b) {
+  if (!b) return QTNH_EINVAL;
+  (void)hipSetDevice(b->device);
+  (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->capture) (void)hipStreamDestroy(b->capture);
+  if (b->mem) (void)hipFree(b->mem);
+  if (b->desc) (void)hipFree(b->desc);
+  delete b;
   return QTNH_OK;
 }
 

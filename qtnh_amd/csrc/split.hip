@@ -410,7 +410,7 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
   int *eb = ea + M;
   // Padding columns of the slice planes must read as zero.
   if (K2 != 2 * K) {
-    qtnh_status st = check_hip(ctx, hipMemsetAsync(As, 0, S * (a_plane + b_plane), ctx->stream), "split zero");
+    qtnh_status st = launch_zero(ctx, As, S * (a_plane + b_plane));
     if (st) return st;
   }
   hipLaunchKernelGGL(split_rows_kernel, dim3((unsigned)M), dim3(kThreads), 0, ctx->stream, A, M, K, K, 0, S, As, a_plane,

@@ -4,6 +4,8 @@
 // outputs, Box-Muller, and log / sin / cos written with + - * / only (no
 // libm, no FMA: this file is built with -ffp-contract=off), so that seeded
 // inputs are identical on the host and the device.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace qtnh_hip {
@@ -130,6 +132,14 @@ __global__ void __launch_bounds__(kRedThreads) norm2_final(const double *__restr
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// Zero fill in 16-byte words (a kernel rather than a memset node, so that
+// callers may capture it in a graph).
+__global__ void zero_kernel(uint4 *__restrict__ p, uint64_t n16, uint8_t *__restrict__ tail, uint64_t ntail) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i = i0; i < n16; i += (uint64_t)gridDim.x * blockDim.x) p[i] = uint4{0, 0, 0, 0};
+  if (i0 < ntail) tail[i0] = 0;
+}
+
 __global__ void scale_kernel(c128 *__restrict__ x, uint64_t n, double re, double im) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     c128 v = x[i];
@@ -167,6 +177,16 @@ qtnh_status launch_norm2(qtnh_ctx_t ctx, const c128 *x, uint64_t n, double *out)
   if (st) return st;
   *out = ctx->host_scalar[0];
   return QTNH_OK;
+}
+
+qtnh_status launch_zero(qtnh_ctx_t ctx, void *p, uint64_t bytes) {
+  if (bytes == 0) return QTNH_OK;
+  // p is 16-byte aligned (device allocations and the offsets used here are).
+  const uint64_t n16 = bytes / 16, ntail = bytes % 16;
+  const unsigned blocks = grid_for(std::max<uint64_t>(n16, 1), 256, 65536);
+  hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, ctx->stream, (uint4 *)p, n16,
+                     (uint8_t *)p + n16 * 16, ntail);
+  return check_hip(ctx, hipGetLastError(), "zero fill launch");
 }
 
 qtnh_status launch_scale(qtnh_ctx_t ctx, c128 *x, uint64_t n, double re, double im) {

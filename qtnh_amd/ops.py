@@ -234,6 +234,15 @@ def tune_circuit(tile_bits=12, low_bits=4):
     lib().qtnh_tune_circuit(tile_bits, low_bits)
 
 
+def autotune_circuit(n_qubits, device=None):
+    """Pick the fused-run tile shape for n-qubit circuits on this context by timing;
+    returns (tile_bits, low_bits, ms)."""
+    ctx = context(device)
+    tb, lb, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+    check(lib().qtnh_autotune_circuit(ctx, n_qubits, ctypes.byref(tb), ctypes.byref(lb), ctypes.byref(ms)), ctx)
+    return tb.value, lb.value, ms.value
+
+
 class Circuit:
     """A qubit circuit (gate list) applied in place to resident 2^n states.
 
@@ -430,6 +439,42 @@ class Network:
         ctx = context(device)
         check(lib().qtnh_net_run(ctx, self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n), ctx)
         return out[:n]
+
+
+class NetworkBatch:
+    """Networks prepared once for repeated contraction (optionally replayed as one
+    HIP graph); new leaf values go in with set_leaf."""
+
+    def __init__(self, nets, use_graph=True, device=None):
+        self.nets = list(nets)
+        self._ctx = context(device)
+        hs = (ctypes.c_void_p * len(self.nets))(*[n.handle.value for n in self.nets])
+        h = ctypes.c_void_p()
+        check(lib().qtnh_net_batch_create(self._ctx, len(self.nets), hs, int(bool(use_graph)), ctypes.byref(h)),
+              self._ctx)
+        self._h = h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().qtnh_net_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_leaf(self, net, leaf, data):
+        x, xp = _c128_host(data)
+        check(lib().qtnh_net_batch_set_leaf(self._ctx, self._h, net, leaf, xp), self._ctx)
+
+    def run(self):
+        outs = [np.zeros(max(_size(n.result_dims()), 1), dtype=np.complex128) for n in self.nets]
+        ps = (ctypes.POINTER(ctypes.c_double) * len(outs))(*[o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                                                              for o in outs])
+        check(lib().qtnh_net_batch_run(self._ctx, self._h, ps), self._ctx)
+        return [o[:_size(n.result_dims())] for o, n in zip(outs, self.nets)]
 
 
 def run_networks(nets, budget_bytes=1 << 40, device=None):
