@@ -22,6 +22,10 @@ namespace {
 
 constexpr int kGateThreads = 256;
 
+// Most workgroups of the streaming gate and diagonal kernels (they loop over
+// the rest): enough to keep many requests in flight on every CU.
+unsigned g_stream_blocks = 1u << 20;
+
 __device__ inline c128 cmul(c128 a, c128 b) {
   c128 r;
   r.re = a.re * b.re - a.im * b.im;
@@ -36,33 +40,54 @@ struct GateArgs {
   int zpos[KB];                   // gate bit positions, ascending (zeros inserted here)
 };
 
-template <int KB>
+typedef double double2_t __attribute__((ext_vector_type(2)));
+
+// U groups per thread and iteration, all loaded before any is stored: more
+// memory requests in flight per thread.  NT: non-temporal loads and stores,
+// used when the lowest gate bit is >= 3, so that the 64 lanes of each load
+// cover whole 128-byte lines (with lower gate bits the lanes of one load are
+// strided inside lines, and the next load of the same lines must hit in cache).
+template <int KB, bool NT>
 __global__ void __launch_bounds__(kGateThreads) gate_bits_kernel(c128 *__restrict__ psi, GateArgs<KB> args,
                                                                  uint64_t n_groups) {
   constexpr int K = 1 << KB;
-  for (uint64_t g = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; g < n_groups;
-       g += (uint64_t)gridDim.x * kGateThreads) {
-    uint64_t base = g;
+  constexpr int U = KB <= 2 ? 2 : 1;
+  double2_t *p2 = reinterpret_cast<double2_t *>(psi);
+  const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
+  for (uint64_t g0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; g0 < n_groups; g0 += stride * U) {
+    uint64_t base[U];
+    double2_t in[U][K];
 #pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      const int p = args.zpos[j];
-      base = ((base >> p) << (p + 1)) | (base & ((1ull << p) - 1));
-    }
-    c128 in[K];
+    for (int u = 0; u < U; ++u) {
+      const uint64_t g = g0 + u * stride;
+      uint64_t b = g < n_groups ? g : g0;
 #pragma unroll
-    for (int c = 0; c < K; ++c) in[c] = psi[base + args.comb[c]];
-#pragma unroll
-    for (int o = 0; o < K; ++o) {
-      c128 acc;
-      acc.re = 0.0;
-      acc.im = 0.0;
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        c128 p = cmul(in[i], args.g[i * K + o]);
-        acc.re = acc.re + p.re;
-        acc.im = acc.im + p.im;
+      for (int j = 0; j < KB; ++j) {
+        const int p = args.zpos[j];
+        b = ((b >> p) << (p + 1)) | (b & ((1ull << p) - 1));
       }
-      psi[base + args.comb[o]] = acc;
+      base[u] = b;
+#pragma unroll
+      for (int c = 0; c < K; ++c) in[u][c] = NT ? __builtin_nontemporal_load(p2 + b + args.comb[c]) : p2[b + args.comb[c]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (g0 + u * stride >= n_groups) continue;
+#pragma unroll
+      for (int o = 0; o < K; ++o) {
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const c128 x = {in[u][i].x, in[u][i].y};
+          const c128 pr = cmul(x, args.g[i * K + o]);
+          ar = ar + pr.re;
+          ai = ai + pr.im;
+        }
+        if (NT)
+          __builtin_nontemporal_store(double2_t{ar, ai}, p2 + base[u] + args.comb[o]);
+        else
+          p2[base[u] + args.comb[o]] = double2_t{ar, ai};
+      }
     }
   }
 }
@@ -76,10 +101,25 @@ struct DiagArgs {
 };
 
 __global__ void __launch_bounds__(kGateThreads) diag_bits_kernel(c128 *__restrict__ psi, DiagArgs args, uint64_t n) {
-  for (uint64_t e = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; e < n; e += (uint64_t)gridDim.x * kGateThreads) {
-    uint32_t c = 0;
-    for (int j = 0; j < args.kb; ++j) c = (c << 1) | (uint32_t)((e >> args.bitpos[j]) & 1ull);
-    psi[e] = cmul(psi[e], args.d[c]);
+  constexpr int U = 4;
+  double2_t *p2 = reinterpret_cast<double2_t *>(psi);
+  const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
+  for (uint64_t e0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; e0 < n; e0 += stride * U) {
+    double2_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t e = e0 + u * stride;
+      v[u] = e < n ? __builtin_nontemporal_load(p2 + e) : double2_t{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t e = e0 + u * stride;
+      if (e >= n) continue;
+      uint32_t c = 0;
+      for (int j = 0; j < args.kb; ++j) c = (c << 1) | (uint32_t)((e >> args.bitpos[j]) & 1ull);
+      const c128 r = cmul(c128{v[u].x, v[u].y}, args.d[c]);
+      __builtin_nontemporal_store(double2_t{r.re, r.im}, p2 + e);
+    }
   }
 }
 
@@ -98,6 +138,122 @@ __global__ void diag_general_kernel(c128 *__restrict__ psi, DiagGenArgs args, ui
   }
 }
 
+// Gates with bits among the lowest 6 ("lane bits": the element's position in
+// a wave's 64 consecutive elements).  Thread t handles the elements whose
+// non-gate bits and lane gate bits come from t (consecutive threads,
+// consecutive elements) and loads one element per combination of the high
+// gate bits: every load and store of a wave covers 1 KiB contiguous, streamed
+// non-temporally.  The inputs held by partner lanes (other lane-bit values)
+// arrive by cross-lane shuffles; each lane computes only its own outputs, with
+// the same sum order and roundings as gate_bits_kernel.
+template <int KB, int KH>
+struct LaneGateArgs {
+  c128 g[(1 << KB) * (1 << KB)];  // G[i][o]
+  uint64_t hoff[1 << KH];         // element offset of high-bit combination h
+  int hzpos[KH > 0 ? KH : 1];     // high gate bits, ascending (zeros inserted here)
+  uint32_t lane_mask;             // lane gate bits
+  uint8_t src_lbits[1 << KB];     // input i: its lane gate bits
+  uint8_t src_h[1 << KB];         // input i: its high combination
+  int8_t jpos[KB];                // gate index bit KB-1-j: state bit
+  int8_t jh[KB];                  // ... its bit in the high combination (-1: a lane bit)
+};
+
+template <int KB, int KH>
+__global__ void __launch_bounds__(kGateThreads) gate_lanes_kernel(c128 *__restrict__ psi, LaneGateArgs<KB, KH> args,
+                                                                  uint64_t n_rows) {
+  constexpr int K = 1 << KB, H = 1 << KH;
+  double2_t *p2 = reinterpret_cast<double2_t *>(psi);
+  const int lane = threadIdx.x & 63;
+  const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
+  // Every lane of a wave takes part in the shuffles: the loop bound is wave-uniform
+  // (n_rows is a multiple of 64 and so is the stride).
+  for (uint64_t t = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; t < n_rows; t += stride) {
+    uint64_t e = t;
+#pragma unroll
+    for (int j = 0; j < KH; ++j) {
+      const int p = args.hzpos[j];
+      e = ((e >> p) << (p + 1)) | (e & ((1ull << p) - 1));
+    }
+    double2_t x[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) x[h] = __builtin_nontemporal_load(p2 + e + args.hoff[h]);
+    // All K inputs of this lane's groups: input i sits in lane (lane & ~mask) | lbits(i), slot h(i).
+    double2_t in[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int src = (lane & ~(int)args.lane_mask) | args.src_lbits[i];
+      const double2_t v = x[args.src_h[i]];
+      in[i].x = __shfl(v.x, src, 64);
+      in[i].y = __shfl(v.y, src, 64);
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      // This lane's output index for high combination h.
+      int o = 0;
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        const int bit = args.jh[j] >= 0 ? ((h >> args.jh[j]) & 1) : ((lane >> args.jpos[j]) & 1);
+        o |= bit << (KB - 1 - j);
+      }
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const c128 xi = {in[i].x, in[i].y};
+        const c128 pr = cmul(xi, args.g[i * K + o]);
+        ar = ar + pr.re;
+        ai = ai + pr.im;
+      }
+      __builtin_nontemporal_store(double2_t{ar, ai}, p2 + e + args.hoff[h]);
+    }
+  }
+}
+
+template <int KB, int KH>
+qtnh_status launch_gate_lanes(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *msb_bits, const double *G) {
+  constexpr int K = 1 << KB;
+  LaneGateArgs<KB, KH> args;
+  memset(&args, 0, sizeof(args));
+  memcpy(args.g, G, sizeof(c128) * K * K);
+  int hbits[KB], nh = 0;
+  for (int j = 0; j < KB; ++j) {
+    args.jpos[j] = (int8_t)msb_bits[j];
+    if (msb_bits[j] < 6) {
+      args.lane_mask |= 1u << msb_bits[j];
+      args.jh[j] = -1;
+    } else {
+      hbits[nh] = msb_bits[j];
+      args.jh[j] = (int8_t)nh++;
+    }
+  }
+  for (int h = 0; h < (1 << KH); ++h) {
+    uint64_t off = 0;
+    for (int q = 0; q < KH; ++q)
+      if ((h >> q) & 1) off |= 1ull << hbits[q];
+    args.hoff[h] = off;
+  }
+  int z[KH > 0 ? KH : 1];
+  for (int q = 0; q < KH; ++q) z[q] = hbits[q];
+  std::sort(z, z + KH);
+  for (int q = 0; q < KH; ++q) args.hzpos[q] = z[q];
+  for (int i = 0; i < K; ++i) {
+    uint32_t lb = 0;
+    int h = 0;
+    for (int j = 0; j < KB; ++j) {
+      if (!((i >> (KB - 1 - j)) & 1)) continue;
+      if (args.jh[j] < 0)
+        lb |= 1u << msb_bits[j];
+      else
+        h |= 1 << args.jh[j];
+    }
+    args.src_lbits[i] = (uint8_t)lb;
+    args.src_h[i] = (uint8_t)h;
+  }
+  const uint64_t rows = n >> KH;
+  unsigned blocks = grid_for(rows, kGateThreads, g_stream_blocks);
+  hipLaunchKernelGGL((gate_lanes_kernel<KB, KH>), dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, rows);
+  return check_hip(ctx, hipGetLastError(), "gate launch");
+}
+
 template <int KB>
 qtnh_status launch_gate_bits(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *msb_bits, const double *G) {
   constexpr int K = 1 << KB;
@@ -114,13 +270,31 @@ qtnh_status launch_gate_bits(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *m
   for (int j = 0; j < KB; ++j) z[j] = msb_bits[j];
   std::sort(z, z + KB);
   for (int j = 0; j < KB; ++j) args.zpos[j] = z[j];
+  // A gate bit among the lowest 6 of a large state: the lane-exchange kernel.
+  int nlow = 0;
+  for (int j = 0; j < KB; ++j) nlow += msb_bits[j] < 6;
+  if (nlow > 0 && n >= 4096) {
+    const int kh = KB - nlow;
+    if (kh == 0) return launch_gate_lanes<KB, 0>(ctx, psi, n, msb_bits, G);
+    if (kh == 1) return launch_gate_lanes<KB, (KB >= 2 ? 1 : 0)>(ctx, psi, n, msb_bits, G);
+    return launch_gate_lanes<KB, (KB >= 3 ? 2 : 0)>(ctx, psi, n, msb_bits, G);
+  }
   const uint64_t groups = n >> KB;
-  unsigned blocks = grid_for(groups, kGateThreads, 1u << 20);
-  hipLaunchKernelGGL(gate_bits_kernel<KB>, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, groups);
+  constexpr int U = KB <= 2 ? 2 : 1;
+  unsigned blocks = grid_for((groups + U - 1) / U, kGateThreads, g_stream_blocks);
+  if (args.zpos[0] >= 3)
+    hipLaunchKernelGGL((gate_bits_kernel<KB, true>), dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, groups);
+  else
+    hipLaunchKernelGGL((gate_bits_kernel<KB, false>), dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args,
+                       groups);
   return check_hip(ctx, hipGetLastError(), "gate launch");
 }
 
 }  // namespace
+
+extern "C" void qtnh_tune_stream(int max_blocks) {
+  g_stream_blocks = (unsigned)std::max(256, std::min(max_blocks, 1 << 22));
+}
 
 qtnh_status launch_gate(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k, const uint16_t *axes,
                         const double *G) {
@@ -186,7 +360,7 @@ qtnh_status launch_diag(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *d
     for (int m = 0; m < k; ++m)
       for (int t = L.width[axes[m]] - 1; t >= 0; --t) args.bitpos[kb++] = L.shift[axes[m]] + t;
     args.kb = kb;
-    unsigned blocks = grid_for(n, kGateThreads, 1u << 20);
+    unsigned blocks = grid_for((n + 3) / 4, kGateThreads, g_stream_blocks);
     hipLaunchKernelGGL(diag_bits_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, state, args, n);
     return check_hip(ctx, hipGetLastError(), "diag launch");
   }

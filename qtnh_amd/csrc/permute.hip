@@ -46,6 +46,12 @@ __device__ inline void store_d2(c128 *p, double2_t v) {
 }
 
 template <bool NT>
+__device__ inline double2_t load_d2(const double2_t *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+template <bool NT>
 __device__ inline void store_c128(c128 *p, c128 v) {
   if (NT) {
     double2_t w = {v.re, v.im};
@@ -113,7 +119,7 @@ permute_bits_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, PermBi
   if (live) {
     const c128 *s = src + xb;
 #pragma unroll
-    for (int r = 0; r < EPT; ++r) v[r] = *at(s, xo[r]);
+    for (int r = 0; r < EPT; ++r) v[r] = load_d2<NT>(at(s, xo[r]));
   }
   while (true) {
     if (live) {
@@ -129,7 +135,7 @@ permute_bits_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, PermBi
       if (live) {
         const c128 *s = src + xn;
 #pragma unroll
-        for (int r = 0; r < EPT; ++r) v[r] = *at(s, xo[r]);
+        for (int r = 0; r < EPT; ++r) v[r] = load_d2<NT>(at(s, xo[r]));
       }
     }
     if (live) {

@@ -228,6 +228,11 @@ def tune_split(slices=5):
     lib().qtnh_tune_split(slices)
 
 
+def tune_stream(max_blocks=1 << 20):
+    """Most workgroups of the streaming gate and diagonal kernels (process-wide)."""
+    lib().qtnh_tune_stream(max_blocks)
+
+
 def tune_circuit(tile_bits=12, low_bits=4):
     """Tile bits (3..13) of fused circuit runs and the lowest state bits every
     tile keeps (0..8), process-wide; results do not depend on them."""
