@@ -159,14 +159,19 @@ def bench_c1(torch, ops, _lib):
 def bench_c3_circuit(torch, ops, _lib, n, depth, seed):
     """Random circuit: every layer a Haar SU(2) on each qubit + CZ on a seeded
     random pairing; one launch per gate vs fused LDS-tiled runs (bit-identical)."""
+    tb, lb, _ = ops.autotune_circuit(n)
     c = ops.Circuit(n).random(depth, seed)
-    _, nops, nruns = c.info()
+    _, nops, _ = c.info()
     N = 1 << n
     a = ops.fill_(ops.empty(N), seed)
     ops.scale_(a, 1 / math.sqrt(ops.norm2(a)))
     b = a.clone()
-    out = {"qubits": n, "depth": depth, "gates": nops, "fused_runs": nruns,
-           "state_GiB": round(N * 16 / 2 ** 30, 2)}
+    out = {"qubits": n, "depth": depth, "gates": nops, "state_GiB": round(N * 16 / 2 ** 30, 2),
+           "autotuned_tile": {"tile_bits": tb, "low_bits": lb}}
+    ops.tune_circuit(tb, lb)  # the run count below reflects the tuned tile
+    nruns = c.info()[2]
+    ops.tune_circuit()
+    out["fused_runs"] = nruns
     for name, mode, st, passes in (("per_gate", _lib.RUN_PER_GATE, a, nops), ("fused", _lib.RUN_FUSED, b, nruns)):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -249,6 +254,15 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
     t2 = time.perf_counter()
     batch, waves = ops.run_networks(nets)
     t3 = time.perf_counter()
+    # Prepared once, replayed as one HIP graph (repeated evaluation of the same structures).
+    prepared = ops.NetworkBatch(nets, use_graph=True)
+    prepared.run()
+    reps = 3
+    t4 = time.perf_counter()
+    for _ in range(reps):
+        replay = prepared.run()
+    t5 = time.perf_counter()
+    prepared.close()
     flops = sum(i["flops"] for i in info)
     return {"networks": count, "redrawn_over_rank_cap": redrawn, "rank_cap": max_rank,
             "max_rank": max(i["max_rank"] for i in info), "steps": sum(i["steps"] for i in info),
@@ -256,7 +270,10 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
             "one_at_a_time_ms": round((t1 - t0) * 1e3, 1), "batched_ms": round((t3 - t2) * 1e3, 1),
             "speedup": round((t1 - t0) / (t3 - t2), 2), "waves": waves,
             "batched_GFLOP_per_s": round(flops / (t3 - t2) / 1e9, 1),
-            "bitwise_equal": all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, batch))}
+            "graph_replay_ms": round((t5 - t4) / reps * 1e3, 1),
+            "graph_replay_speedup": round((t1 - t0) / ((t5 - t4) / reps), 2),
+            "bitwise_equal": all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, batch))
+            and all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, replay))}
 
 
 def bench_c5(torch, ops, _lib, count, rate, seed):

@@ -281,6 +281,11 @@ void qtnh_tune_stream(int max_blocks);
  * lowest state bits every tile keeps (0..8, default 4; 2^low x 16 bytes are
  * contiguous in each tile row).  Results do not depend on them. */
 void qtnh_tune_circuit(int tile_bits, int low_bits);
+/* Kernel for large states with 12-bit tiles: 2 (default) register-blocked
+ * (ops applied to 16 amplitudes per thread in registers, one LDS round trip
+ * per phase of ops on 4 tile bits), 1 pipelined (next tile streamed into LDS
+ * during the current tile's ops), 0 plain.  Results do not depend on it. */
+void qtnh_tune_circuit_pipeline(int mode);
 
 #ifdef __cplusplus
 }

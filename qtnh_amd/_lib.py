@@ -63,6 +63,7 @@ SIGNATURES = {
     "qtnh_tune_split": [_i],
     "qtnh_tune_circuit": [_i, _i],
     "qtnh_tune_stream": [_i],
+    "qtnh_tune_circuit_pipeline": [_i],
     "qtnh_circuit_create": [_i, ctypes.POINTER(_p)],
     "qtnh_circuit_destroy": [_p],
     "qtnh_circuit_gate": [_p, _i, _u16p, _dp],
@@ -95,7 +96,7 @@ SIGNATURES = {
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_zgemm": None,
             "qtnh_tune_split": None, "qtnh_tune_circuit": None,
-            "qtnh_tune_stream": None}
+            "qtnh_tune_stream": None, "qtnh_tune_circuit_pipeline": None}
 
 _LIB = None
 

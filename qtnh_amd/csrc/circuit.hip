@@ -45,7 +45,16 @@ struct TileOp {
   int pos[kMaxOpQubits];    // tile bit of op-index bit j (j = 0 most significant)
   int zpos[kMaxOpQubits];   // the same tile bits, ascending
   uint32_t coef;            // first coefficient (complex units)
-  uint32_t pad;
+  uint32_t rcode;           // register form (circuit_rb_kernel): kind | kb << 1 | reg bit j << (3 + 2 j)
+};
+
+// A phase of a run: consecutive 1- and 2-qubit ops whose bits lie in 4 tile
+// bits (pmask), applied to 16 amplitudes held in registers by each thread; or
+// (lds = 1) a single 3-qubit op applied in LDS.
+struct PhaseDesc {
+  uint32_t op_begin, op_end;
+  uint32_t pmask;
+  uint32_t lds;
 };
 
 struct RunDesc {
@@ -59,6 +68,8 @@ struct RunDesc {
   uint32_t coef_begin;  // first coefficient of op_begin (ops' coefficients are contiguous)
   int tb;               // tile bits
   int n_outer;          // popcount(outer_mask)
+  const PhaseDesc *phases;  // register phases of this run (12-bit tiles), or null
+  uint32_t ph_begin, ph_end;
 };
 
 constexpr int kChunkOps = 32;     // ops staged in LDS at a time
@@ -249,6 +260,344 @@ __global__ void __launch_bounds__(NT, (NT == 256 && TBMAX < 13) ? 2 : 1)
   }
 }
 
+// Large states: runs whose tiles have exactly kPipeTB bits.  One 512-thread
+// workgroup per CU walks its tiles with two LDS tile buffers: while the ops of
+// tile k run on one buffer, tile k+1 streams into the other straight from HBM
+// (global_load_lds_dwordx4: each wave instruction gathers 64 tile elements,
+// 1 KiB, into consecutive LDS slots), and tile k is written back with
+// non-temporal stores.  Same ops, same order, same arithmetic as
+// circuit_tile_kernel.
+constexpr int kPipeTB = 12, kPipeT = 1 << kPipeTB, kPipeNT = 512;
+constexpr int kPipeWaves = kPipeNT / 64, kPipeCPW = kPipeT / 64 / kPipeWaves;  // 8 chunks of 64 per wave
+
+__device__ __forceinline__ int find_run(const RunDesc *__restrict__ runs, int nruns, uint64_t gt) {
+  int lo = 0, hi = nruns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (runs[mid].first <= gt)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// This lane's element offsets in tile t of run R: element l = 64 (wave + kPipeWaves k) + lane.
+__device__ __forceinline__ double2_t *pipe_offsets(const RunDesc &R, uint64_t t, int wave, int lane,
+                                                   uint64_t (&off)[kPipeCPW]) {
+  uint64_t lo6 = 0, rest = R.tile_mask;
+  for (int b = 0; b < 6; ++b) {
+    const uint64_t low = rest & (0 - rest);
+    lo6 |= low;
+    rest ^= low;
+  }
+  const uint64_t lo = deposit(lane, lo6);
+#pragma unroll
+  for (int k = 0; k < kPipeCPW; ++k) off[k] = lo | deposit(wave + kPipeWaves * k, rest);
+  return R.psi + deposit(t, R.outer_mask);
+}
+
+__device__ __forceinline__ void pipe_issue(double2_t *dstbuf, const double2_t *psi, const uint64_t (&off)[kPipeCPW],
+                                           int wave) {
+#pragma unroll
+  for (int k = 0; k < kPipeCPW; ++k)
+    __builtin_amdgcn_global_load_lds((const void *)(psi + off[k]),
+                                     (void __attribute__((address_space(3))) *)(dstbuf + 64 * (wave + kPipeWaves * k)),
+                                     16, 0, 0);
+}
+
+__global__ void __launch_bounds__(kPipeNT, 1)
+    circuit_pipe_kernel(const RunDesc *__restrict__ runs, int nruns, uint64_t total_tiles) {
+  __shared__ double2_t buf[2][kPipeT];
+  __shared__ double2_t cbuf[kChunkCoef];
+  __shared__ TileOp obuf[kChunkOps];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t gt = blockIdx.x;
+  if (gt >= total_tiles) return;  // uniform: the whole workgroup leaves
+  int r = find_run(runs, nruns, gt);
+  uint64_t off[kPipeCPW];
+  double2_t *psi = pipe_offsets(runs[r], gt - runs[r].first, wave, lane, off);
+  pipe_issue(buf[0], psi, off, wave);
+  int cur = 0;
+  while (true) {
+    const RunDesc &R = runs[r];
+    // This tile's gather is complete (and the previous tile's stores have been
+    // accepted: the counter covers both, and nothing here relies on their order).
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    // Stage the first chunk of ops before the next gather is issued (their
+    // loads then do not wait behind it).
+    uint32_t o = R.op_begin, cbase = R.coef_begin;
+    auto stage = [&](uint32_t &oe, uint32_t &ce) {
+      oe = o;
+      ce = cbase;
+      while (oe < R.op_end && oe - o < (uint32_t)kChunkOps) {
+        const TileOp &x = R.ops[oe];
+        const uint32_t nco = x.kind == 1 ? (1u << x.kb) : (1u << (2 * x.kb));
+        if (x.coef + nco - cbase > (uint32_t)kChunkCoef) break;
+        ce = x.coef + nco;
+        ++oe;
+      }
+      for (uint32_t i = threadIdx.x; i < oe - o; i += kPipeNT) obuf[i] = R.ops[o + i];
+      for (uint32_t i = threadIdx.x; i < ce - cbase; i += kPipeNT) cbuf[i] = R.coef[cbase + i];
+    };
+    uint32_t oe, ce;
+    stage(oe, ce);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const uint64_t gn = gt + gridDim.x;
+    const bool more = gn < total_tiles;  // uniform
+    int rn = r;
+    uint64_t offn[kPipeCPW];
+    double2_t *psin = psi;
+    if (more) {
+      rn = find_run(runs, nruns, gn);
+      psin = pipe_offsets(runs[rn], gn - runs[rn].first, wave, lane, offn);
+      pipe_issue(buf[cur ^ 1], psin, offn, wave);
+    }
+    double2_t *tb = buf[cur];
+    while (true) {
+      for (uint32_t i = 0; i < oe - o; ++i) {
+        const TileOp &op = obuf[i];
+        const double2_t *g = cbuf + (op.coef - cbase);
+        if (op.kind == 1) {
+          if (op.kb == 1)
+            diag_in_lds<1, kPipeNT>(tb, kPipeTB, op, g);
+          else if (op.kb == 2)
+            diag_in_lds<2, kPipeNT>(tb, kPipeTB, op, g);
+          else
+            diag_in_lds<3, kPipeNT>(tb, kPipeTB, op, g);
+        } else if (op.kb == 1) {
+          gate_in_lds<1, kPipeNT>(tb, kPipeTB, op, g);
+        } else if (op.kb == 2) {
+          gate_in_lds<2, kPipeNT>(tb, kPipeTB, op, g);
+        } else {
+          gate_in_lds<3, kPipeNT>(tb, kPipeTB, op, g);
+        }
+        __syncthreads();
+      }
+      o = oe;
+      cbase = ce;
+      if (o >= R.op_end) break;
+      // More ops than one chunk: stage the next chunk (waits behind the gather).
+      stage(oe, ce);
+      __syncthreads();
+    }
+    // Write the tile back.
+#pragma unroll
+    for (int k = 0; k < kPipeCPW; ++k)
+      __builtin_nontemporal_store(tb[64 * (wave + kPipeWaves * k) + lane], psi + off[k]);
+    if (!more) break;
+    gt = gn;
+    r = rn;
+    psi = psin;
+#pragma unroll
+    for (int k = 0; k < kPipeCPW; ++k) off[k] = offn[k];
+    cur ^= 1;
+  }
+}
+
+// Register-blocked runs (12-bit tiles, 256 threads): per phase, each thread
+// takes the 16 tile amplitudes that differ only in the phase's 4 bits from
+// LDS into registers, applies the phase's ops there (compile-time indexing:
+// one instantiation per op shape and register-bit assignment) and writes them
+// back; one LDS round trip and one barrier per phase instead of per op.  The
+// ops do exactly the arithmetic of gate_in_lds / diag_in_lds.  LDS slots are
+// XOR-swizzled (slot = l ^ ((l >> 4) & 15)) so that phases over the low tile
+// bits do not fall on one bank.
+template <int KB, int A, int B, int C>
+__device__ __forceinline__ void reg_gate(double2_t (&v)[16], const double2_t *__restrict__ g) {
+  constexpr int K = 1 << KB;
+  constexpr int M = (1 << A) | (KB > 1 ? (1 << B) : 0) | (KB > 2 ? (1 << C) : 0);
+#pragma unroll
+  for (int r0 = 0; r0 < 16; ++r0) {
+    if (r0 & M) continue;
+    // 3-qubit gates: re-read the 64 coefficients per group instead of holding
+    // them in registers (a compiler barrier, no code).
+    if (KB == 3) asm volatile("" ::: "memory");
+    int idx[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      int x = r0;
+      if ((c >> (KB - 1)) & 1) x |= 1 << A;
+      if (KB > 1 && ((c >> (KB - 2)) & 1)) x |= 1 << B;
+      if (KB > 2 && (c & 1)) x |= 1 << C;
+      idx[c] = x;
+    }
+    double2_t in[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) in[c] = v[idx[c]];
+#pragma unroll
+    for (int o = 0; o < K; ++o) {
+      if (KB == 3) asm volatile("" ::: "memory");  // one output's 8 coefficients live at a time
+      double2_t acc = {0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const double2_t p = cmul2(in[i], g[i * K + o]);
+        acc.x = acc.x + p.x;
+        acc.y = acc.y + p.y;
+      }
+      v[idx[o]] = acc;
+    }
+  }
+}
+
+template <int KB, int A, int B, int C>
+__device__ __forceinline__ void reg_diag(double2_t (&v)[16], const double2_t *__restrict__ d) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    int c = (r >> A) & 1;
+    if (KB > 1) c = (c << 1) | ((r >> B) & 1);
+    if (KB > 2) c = (c << 1) | ((r >> C) & 1);
+    v[r] = cmul2(v[r], d[c]);
+  }
+}
+
+__device__ __forceinline__ void apply_reg_op(double2_t (&v)[16], uint32_t code, const double2_t *__restrict__ g) {
+  switch (code) {
+    case 2: reg_gate<1, 0, 0, 0>(v, g); break;
+    case 10: reg_gate<1, 1, 0, 0>(v, g); break;
+    case 18: reg_gate<1, 2, 0, 0>(v, g); break;
+    case 26: reg_gate<1, 3, 0, 0>(v, g); break;
+    case 36: reg_gate<2, 0, 1, 0>(v, g); break;
+    case 68: reg_gate<2, 0, 2, 0>(v, g); break;
+    case 100: reg_gate<2, 0, 3, 0>(v, g); break;
+    case 12: reg_gate<2, 1, 0, 0>(v, g); break;
+    case 76: reg_gate<2, 1, 2, 0>(v, g); break;
+    case 108: reg_gate<2, 1, 3, 0>(v, g); break;
+    case 20: reg_gate<2, 2, 0, 0>(v, g); break;
+    case 52: reg_gate<2, 2, 1, 0>(v, g); break;
+    case 116: reg_gate<2, 2, 3, 0>(v, g); break;
+    case 28: reg_gate<2, 3, 0, 0>(v, g); break;
+    case 60: reg_gate<2, 3, 1, 0>(v, g); break;
+    case 92: reg_gate<2, 3, 2, 0>(v, g); break;
+    case 3: reg_diag<1, 0, 0, 0>(v, g); break;
+    case 11: reg_diag<1, 1, 0, 0>(v, g); break;
+    case 19: reg_diag<1, 2, 0, 0>(v, g); break;
+    case 27: reg_diag<1, 3, 0, 0>(v, g); break;
+    case 37: reg_diag<2, 0, 1, 0>(v, g); break;
+    case 69: reg_diag<2, 0, 2, 0>(v, g); break;
+    case 101: reg_diag<2, 0, 3, 0>(v, g); break;
+    case 13: reg_diag<2, 1, 0, 0>(v, g); break;
+    case 77: reg_diag<2, 1, 2, 0>(v, g); break;
+    case 109: reg_diag<2, 1, 3, 0>(v, g); break;
+    case 21: reg_diag<2, 2, 0, 0>(v, g); break;
+    case 53: reg_diag<2, 2, 1, 0>(v, g); break;
+    case 117: reg_diag<2, 2, 3, 0>(v, g); break;
+    case 29: reg_diag<2, 3, 0, 0>(v, g); break;
+    case 61: reg_diag<2, 3, 1, 0>(v, g); break;
+    case 93: reg_diag<2, 3, 2, 0>(v, g); break;
+    default: break;
+  }
+}
+
+__device__ __forceinline__ int rb_slot(int l) { return l ^ ((l >> 4) & 15); }
+
+// A 3-qubit op on the swizzled 12-bit tile: the arithmetic of gate_in_lds<3> / diag_in_lds<3>.
+__device__ __forceinline__ void rb_lds_op3(double2_t *buf, const TileOp &op, const double2_t *__restrict__ g) {
+  if (op.kind == 1) {
+    for (int l = threadIdx.x; l < 4096; l += 256) {
+      int c = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) c = (c << 1) | ((l >> op.pos[j]) & 1);
+      buf[rb_slot(l)] = cmul2(buf[rb_slot(l)], g[c]);
+    }
+    return;
+  }
+  int comb[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    int off = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if ((c >> (2 - j)) & 1) off |= 1 << op.pos[j];
+    comb[c] = off;
+  }
+  for (int gi = threadIdx.x; gi < 512; gi += 256) {
+    asm volatile("" ::: "memory");  // coefficients re-read per group, not held in registers
+    int base = gi;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int p = op.zpos[j];
+      base = ((base >> p) << (p + 1)) | (base & ((1 << p) - 1));
+    }
+    double2_t in[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) in[c] = buf[rb_slot(base + comb[c])];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      double2_t acc = {0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const double2_t p = cmul2(in[i], g[i * 8 + o]);
+        acc.x = acc.x + p.x;
+        acc.y = acc.y + p.y;
+      }
+      buf[rb_slot(base + comb[o])] = acc;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256, 2) circuit_rb_kernel(const RunDesc *__restrict__ runs, int nruns,
+                                                            uint64_t total_tiles) {
+  __shared__ double2_t buf[1 << 12];
+  const int tid = threadIdx.x;
+  for (uint64_t gt = blockIdx.x; gt < total_tiles; gt += gridDim.x) {
+    const RunDesc &R = runs[find_run(runs, nruns, gt)];
+    const uint64_t t = gt - R.first;
+    // Gather: element l = tid + 256 j (its low 8 tile bits are the thread's).
+    uint64_t lo_mask = 0, rest = R.tile_mask;
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t low = rest & (0 - rest);
+      lo_mask |= low;
+      rest ^= low;
+    }
+    double2_t *psi = R.psi + deposit(t, R.outer_mask);
+    const uint64_t lo_off = deposit(tid, lo_mask);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) buf[rb_slot(tid + 256 * j)] = __builtin_nontemporal_load(psi + (lo_off | deposit(j, rest)));
+    __syncthreads();
+    for (uint32_t ph = R.ph_begin; ph < R.ph_end; ++ph) {
+      const PhaseDesc P = R.phases[ph];
+      if (P.lds) {
+        // A 3-qubit op, in LDS (its tile bits index the swizzled buffer through rb_lds).
+        const TileOp op = R.ops[P.op_begin];
+        rb_lds_op3(buf, op, R.coef + op.coef);
+        __syncthreads();
+        continue;
+      }
+      const uint32_t comp = 0xFFFu & ~P.pmask;
+      const int base = (int)deposit(tid, comp);
+      // The phase's 4 tile bits as values: register index r sets bit k -> tile bit pb[k].
+      int pb[4];
+      uint32_t pm = P.pmask;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        pb[k] = (int)(pm & (0u - pm));
+        pm ^= (uint32_t)pb[k];
+      }
+      int slot[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        slot[r] = rb_slot(base | ((r & 1) ? pb[0] : 0) | ((r & 2) ? pb[1] : 0) | ((r & 4) ? pb[2] : 0) |
+                          ((r & 8) ? pb[3] : 0));
+      double2_t v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = buf[slot[r]];
+      for (uint32_t o = P.op_begin; o < P.op_end; ++o) {
+        const TileOp op = R.ops[o];
+        apply_reg_op(v, op.rcode, R.coef + op.coef);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) buf[slot[r]] = v[r];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) __builtin_nontemporal_store(buf[rb_slot(tid + 256 * j)], psi + (lo_off | deposit(j, rest)));
+    __syncthreads();
+  }
+}
+
 // ---- <Z_q> for a list of qubits --------------------------------------------
 // Chunks of 2^kChunkBits consecutive amplitudes.  Per chunk: the total
 // probability and, for each of the chunk's own index bits, the probability
@@ -355,12 +704,17 @@ struct qtnh_circuit_s {
   };
   std::vector<Run> runs;
   std::vector<TileOp> tops;  // host copy of the compiled ops
+  qtnh_hip::PhaseDesc *d_phases = nullptr;
+  std::vector<std::pair<uint32_t, uint32_t>> run_phases;  // per run: [begin, end) into the phases
 };
 
 namespace qtnh_hip {
 namespace {
 
 int g_tile_bits = 12;
+// Large states with 12-bit tiles: 2 register-blocked kernel, 1 pipelined LDS
+// kernel, 0 plain LDS kernel.
+int g_circuit_mode = 2;
 int g_low_bits = 4;  // lowest state bits in every tile: 2^4 x 16 B = 256 B contiguous rows
 
 uint64_t mask_of(const qtnh_circuit_s &c, const qtnh_circuit_s::Op &op) {
@@ -403,9 +757,69 @@ void plan_runs(qtnh_circuit_s &c, int tb, int low_bits) {
 void free_compiled(qtnh_circuit_s &c) {
   if (c.d_ops) (void)hipFree(c.d_ops);
   if (c.d_coef) (void)hipFree(c.d_coef);
+  if (c.d_phases) (void)hipFree(c.d_phases);
   c.d_ops = nullptr;
   c.d_coef = nullptr;
+  c.d_phases = nullptr;
   c.compiled_tb = -1;
+}
+
+// Register phases of a 12-bit-tile run: consecutive ops whose tile bits fit 4,
+// padded with the highest free tile bits (threads then vary in the low bits).
+void plan_phases(std::vector<TileOp> &tops, uint32_t begin, uint32_t end, std::vector<PhaseDesc> &out) {
+  uint32_t cur = 0, pb = begin;
+  auto mask_of_op = [&](const TileOp &t) {
+    uint32_t m = 0;
+    for (int j = 0; j < t.kb; ++j) m |= 1u << t.pos[j];
+    return m;
+  };
+  auto close = [&](uint32_t pe) {
+    if (pe == pb) return;
+    if (tops[pb].kb == 3) {
+      PhaseDesc ph;
+      ph.op_begin = pb;
+      ph.op_end = pe;
+      ph.pmask = 0;
+      ph.lds = 1;
+      tops[pb].rcode = 0;
+      out.push_back(ph);
+      return;
+    }
+    uint32_t m = cur;
+    for (int b = 11; b >= 0 && __builtin_popcount(m) < 4; --b) m |= 1u << b;
+    PhaseDesc ph;
+    ph.op_begin = pb;
+    ph.op_end = pe;
+    ph.pmask = m;
+    ph.lds = 0;
+    // Register bit of tile bit p: its rank among the phase's 4 bits.
+    auto regbit = [&](int p) { return __builtin_popcount(m & ((1u << p) - 1)); };
+    for (uint32_t i = pb; i < pe; ++i) {
+      TileOp &t = tops[i];
+      uint32_t code = (uint32_t)t.kind | ((uint32_t)t.kb << 1);
+      for (int j = 0; j < t.kb; ++j) code |= (uint32_t)regbit(t.pos[j]) << (3 + 2 * j);
+      t.rcode = code;
+    }
+    out.push_back(ph);
+  };
+  for (uint32_t i = begin; i < end; ++i) {
+    const uint32_t m = mask_of_op(tops[i]);
+    if (tops[i].kb == 3) {
+      close(i);
+      pb = i;
+      close(i + 1);
+      pb = i + 1;
+      cur = 0;
+      continue;
+    }
+    if (__builtin_popcount(cur | m) > 4) {
+      close(i);
+      pb = i;
+      cur = 0;
+    }
+    cur |= m;
+  }
+  close(end);
 }
 
 // Tile shape for an n-qubit circuit: the autotuned one of this context, else the process default.
@@ -446,7 +860,20 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
     }
   }
   if (coef.empty()) coef.assign(2, 0.0);
+  std::vector<PhaseDesc> phases;
+  c.run_phases.clear();
+  for (const auto &r : c.runs) {
+    const uint32_t p0 = (uint32_t)phases.size();
+    if (r.tbits.size() == 12) plan_phases(tops, r.begin, r.end, phases);
+    c.run_phases.push_back({p0, (uint32_t)phases.size()});
+  }
+  if (phases.empty()) phases.resize(1);
   qtnh_status st = check_hip(ctx, hipMalloc((void **)&c.d_ops, tops.size() * sizeof(TileOp)), "circuit alloc");
+  if (!st) st = check_hip(ctx, hipMalloc((void **)&c.d_phases, phases.size() * sizeof(PhaseDesc)), "circuit alloc");
+  if (!st)
+    st = check_hip(ctx, hipMemcpyAsync(c.d_phases, phases.data(), phases.size() * sizeof(PhaseDesc),
+                                       hipMemcpyHostToDevice, ctx->stream),
+                   "circuit upload");
   if (!st) st = check_hip(ctx, hipMalloc((void **)&c.d_coef, coef.size() * sizeof(double)), "circuit alloc");
   if (!st)
     st = check_hip(ctx, hipMemcpyAsync(c.d_ops, tops.data(), tops.size() * sizeof(TileOp), hipMemcpyHostToDevice,
@@ -467,9 +894,13 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
   return QTNH_OK;
 }
 
-RunDesc make_desc(const qtnh_circuit_s &c, const qtnh_circuit_s::Run &r, c128 *psi, uint64_t first) {
+RunDesc make_desc(const qtnh_circuit_s &c, size_t ri, c128 *psi, uint64_t first) {
+  const qtnh_circuit_s::Run &r = c.runs[ri];
   RunDesc d;
   memset(&d, 0, sizeof(d));
+  d.phases = c.d_phases;
+  d.ph_begin = c.run_phases[ri].first;
+  d.ph_end = c.run_phases[ri].second;
   d.ops = c.d_ops;
   d.coef = c.d_coef;
   d.psi = (double2_t *)psi;
@@ -510,6 +941,22 @@ qtnh_status launch_run_groups(qtnh_ctx_t ctx, const std::vector<RunDesc> &descs,
       total = descs[i].first + desc_tiles(descs[i]);
     }
     const int nr = (int)(e - b);
+    bool all12 = true;
+    for (size_t i = b; i < e; ++i) all12 = all12 && descs[i].tb == kPipeTB && descs[i].ph_end > descs[i].ph_begin;
+    if (all12 && total >= 2ull * ctx->num_cus && g_circuit_mode == 2) {
+      const unsigned rbk = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * 2);
+      hipLaunchKernelGGL(circuit_rb_kernel, dim3(rbk), dim3(256), 0, ctx->stream, dd + b, nr, total);
+      st = check_hip(ctx, hipGetLastError(), "circuit register-blocked launch");
+      if (st) return st;
+      continue;
+    }
+    if (all12 && total >= 4ull * ctx->num_cus && g_circuit_mode == 1) {
+      const unsigned pb = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus);
+      hipLaunchKernelGGL(circuit_pipe_kernel, dim3(pb), dim3(kPipeNT), 0, ctx->stream, dd + b, nr, total);
+      st = check_hip(ctx, hipGetLastError(), "circuit pipe launch");
+      if (st) return st;
+      continue;
+    }
     // Few tiles: 1024 threads per tile; many: 256 threads, several tiles per CU.
     const bool wide = total < (uint64_t)ctx->num_cus;
     const unsigned per_cu = wide ? 1 : tbmax <= 10 ? 8 : tbmax == 11 ? 4 : tbmax == 12 ? 2 : 1;
@@ -540,9 +987,9 @@ qtnh_status run_fused(qtnh_ctx_t ctx, qtnh_circuit_s &c, c128 *psi) {
   if (st) return st;
   std::vector<RunDesc> descs;
   std::vector<size_t> starts;
-  for (const auto &r : c.runs) {
+  for (size_t ri = 0; ri < c.runs.size(); ++ri) {
     starts.push_back(descs.size());
-    descs.push_back(make_desc(c, r, psi, 0));
+    descs.push_back(make_desc(c, ri, psi, 0));
   }
   starts.push_back(descs.size());
   return launch_run_groups(ctx, descs, starts);
@@ -652,6 +1099,8 @@ void qtnh_tune_circuit(int tile_bits, int low_bits) {
   g_tile_bits = std::max(3, std::min(tile_bits, kMaxTileBits));
   g_low_bits = std::max(0, std::min(low_bits, 8));
 }
+
+void qtnh_tune_circuit_pipeline(int mode) { g_circuit_mode = std::max(0, std::min(mode, 2)); }
 
 qtnh_status qtnh_circuit_create(int n_qubits, qtnh_circuit_t *out) {
   if (!out) return QTNH_EINVAL;
@@ -839,7 +1288,7 @@ qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit
     uint64_t first = 0;
     for (int i = 0; i < count; ++i) {
       if (s >= cs[i]->runs.size()) continue;
-      RunDesc d = make_desc(*cs[i], cs[i]->runs[s], (c128 *)states[i], first);
+      RunDesc d = make_desc(*cs[i], s, (c128 *)states[i], first);
       first += desc_tiles(d);
       descs.push_back(d);
     }

@@ -248,6 +248,12 @@ def autotune_circuit(n_qubits, device=None):
     return tb.value, lb.value, ms.value
 
 
+def tune_circuit_pipeline(mode=2):
+    """Fused kernel for large states: 2 register-blocked (default), 1 pipelined, 0 plain;
+    results do not depend on it."""
+    lib().qtnh_tune_circuit_pipeline(int(mode))
+
+
 class Circuit:
     """A qubit circuit (gate list) applied in place to resident 2^n states.
 

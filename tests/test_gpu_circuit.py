@@ -107,6 +107,28 @@ def test_every_tile_size_bitexact(tb):
         ops.tune_circuit(12)
 
 
+@pytest.mark.parametrize("mode", [2, 1, 0])
+def test_large_state_kernels_bitexact(mode):
+    """22 qubits: 1024 tiles of 12 bits.  Register-blocked (2), pipelined (1)
+    and plain (0) fused kernels all equal the oracle; the circuit mixes 1-, 2-
+    and 3-qubit gates and diagonals on low and high qubits."""
+    n = 22
+    c = ops.Circuit(n).random(2, 2222)
+    rng = np.random.default_rng(mode)
+    g3 = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    c.gate([21, 20, 19], g3).diag([0, 21, 10], np.exp(1j * rng.standard_normal(8))).gate([20, 3], rng.standard_normal(16))
+    c.gate([21], [0, 1, 1, 0]).diag([18, 21], [1, 1j, -1, -1j])
+    x = rnd_state(n, 22)
+    ref = oracle_run(c, x)
+    ops.tune_circuit_pipeline(mode)
+    try:
+        s = dev(x)
+        c.run(s)
+        assert same_bits(host(s), ref), mode
+    finally:
+        ops.tune_circuit_pipeline(2)
+
+
 def test_batch_equals_individual_runs():
     sizes = [3, 9, 12, 13, 14, 16, 12, 5]
     circuits = [ops.Circuit(n).random(3, 40 + i) for i, n in enumerate(sizes)]
