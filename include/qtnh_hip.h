@@ -266,9 +266,16 @@ qtnh_status qtnh_expect_z_c128(qtnh_ctx_t ctx, const void *state, int n_qubits, 
 /* Process-wide tile parameters of the power-of-two permute kernel: starting
  * source-low / result-low tile bits a, b (1..12), target tile bits (<= 12),
  * tile walk order (0: source order, 1: page-local on both sides), non-temporal
- * stores (0/1), persistent blocks per compute unit.  Results do not depend on
- * them. */
+ * loads and stores (0/1), persistent blocks per compute unit.  Results do not
+ * depend on them.  Setting them turns per-permutation tuning off. */
 void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks_per_cu);
+/* Per-permutation tuning of the power-of-two permute (on by default): the
+ * first permute of >= 2^23 elements with a new bit permutation on a device
+ * times a few tile shapes on the caller's buffers (one stream synchronisation;
+ * skipped while the stream is being captured) and keeps the fastest.  on = 1
+ * restores the default tile and clears the tuned shapes; 0 uses the set tile
+ * for every permutation.  Results do not depend on it. */
+void qtnh_tune_permute_auto(int on);
 /* Block width (64 or 128 complex columns, 0: chosen by size) of the FP64
  * complex GEMM. */
 void qtnh_tune_zgemm(int bn);

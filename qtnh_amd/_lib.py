@@ -59,6 +59,7 @@ SIGNATURES = {
     "qtnh_norm2_c128": [_p, _p, _u64, _dp],
     "qtnh_scale_c128": [_p, _p, _u64, ctypes.c_double, ctypes.c_double],
     "qtnh_tune_permute": [_i, _i, _i, _i, _i, _i],
+    "qtnh_tune_permute_auto": [_i],
     "qtnh_tune_zgemm": [_i],
     "qtnh_tune_split": [_i],
     "qtnh_tune_circuit": [_i, _i],
@@ -94,7 +95,7 @@ SIGNATURES = {
     "qtnh_autotune_circuit": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _dp],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
-            "qtnh_tune_permute": None, "qtnh_tune_zgemm": None,
+            "qtnh_tune_permute": None, "qtnh_tune_permute_auto": None, "qtnh_tune_zgemm": None,
             "qtnh_tune_split": None, "qtnh_tune_circuit": None,
             "qtnh_tune_stream": None, "qtnh_tune_circuit_pipeline": None}
 

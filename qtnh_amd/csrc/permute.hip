@@ -14,6 +14,9 @@
 // Other dims: one thread per result element with mixed-radix decoding.
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "internal.hpp"
 
@@ -212,14 +215,32 @@ __global__ void swap_general_kernel(c128 *__restrict__ buf, uint64_t n, uint64_t
 // Tunables of the bit-permute tile.  a / b: starting source-low / result-low
 // bits; the tile then grows until it holds u_target bits (<= 12, 64 KiB of
 // LDS).  order 0 walks tiles by source bit position, 1 by the larger of the
-// source and result positions (page locality on both sides).  nt: stores
-// bypass the caches.  blocks_per_cu: persistent blocks per compute unit.
-// Defaults measured on MI355X (rank-28 tensors): 5.1-5.2 TB/s over random,
-// reversed, low-bits-kept and bit-0-to-top permutations.
+// source and result positions (page locality on both sides).  nt: loads and
+// stores bypass the caches.  blocks_per_cu: blocks per compute unit, each
+// looping over tiles.
 struct PermTune {
   int a = 6, b = 6, u_target = 12, order = 1, nt = 1, blocks_per_cu = 2;
 };
 static PermTune g_perm;
+
+// Per-permutation tuning.  No one tile shape is best for every permutation on
+// MI355X (rank-28, TB/s: the default 12-bit tile with two blocks per CU gives
+// 4.5-5.3; an 11-bit tile with 64-128 blocks per CU, each looping over a few
+// tiles, gives 5.9-6.0 on some permutations and 4.7 on others), and every
+// shape writes the same bits.  So the first permute of a
+// large state (>= 2^23 elements) with a new (bit count, bit permutation)
+// signature on a device times each candidate below on the caller's buffers
+// and keeps the fastest for that signature.  The timing synchronises the
+// stream once; it is skipped (defaults used, nothing cached) while the stream
+// is being captured into a graph.
+static bool g_perm_auto = true;
+static const PermTune kPermCandidates[] = {{6, 6, 12, 1, 1, 2},  {6, 6, 12, 1, 1, 128}, {5, 6, 11, 0, 1, 64},
+                                           {5, 6, 11, 1, 1, 128}, {5, 6, 11, 1, 1, 32},  {6, 5, 11, 0, 1, 256}};
+constexpr int kPermTimedRuns = 3;
+constexpr uint64_t kPermAutoMin = 1ull << 23;
+constexpr size_t kPermCacheMax = 4096;
+static std::mutex g_perm_mu;
+static std::map<std::string, PermTune> g_perm_cache;
 
 extern "C" void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks_per_cu) {
   g_perm.a = std::max(1, std::min(a, 12));
@@ -228,6 +249,14 @@ extern "C" void qtnh_tune_permute(int a, int b, int u_target, int order, int nt,
   g_perm.order = order;
   g_perm.nt = nt;
   g_perm.blocks_per_cu = std::max(1, blocks_per_cu);
+  g_perm_auto = false;
+}
+
+extern "C" void qtnh_tune_permute_auto(int on) {
+  std::lock_guard<std::mutex> lock(g_perm_mu);
+  g_perm_auto = on != 0;
+  if (on) g_perm = PermTune{};
+  g_perm_cache.clear();
 }
 
 static qtnh_status launch_copy(qtnh_ctx_t ctx, const c128 *src, c128 *dst, uint64_t n) {
@@ -274,6 +303,145 @@ bool tile_ok(const int *inv, int a, int b, int nb) {
   int u = tile_bits(inv, a, b);
   return a <= nb && b <= nb && u <= 12 && u - a <= 6 && u - b <= 6;
 }
+// Tile arguments of the bit permutation sigma (source bit -> result bit,
+// inv its inverse) for the tile shape cfg.  False on an internal mismatch.
+bool plan_bits(const int *sigma, const int *inv, int nb, const PermTune &cfg, PermBitsArgs &args, int *u_out,
+               int *no_out) {
+  memset(&args, 0, sizeof(args));
+  int a = std::min(cfg.a, nb), b = std::min(cfg.b, nb);
+  if (!tile_ok(inv, a, b, nb)) a = b = std::min(6, nb);
+  // Grow the tile while it is smaller than the target (low bits kept in place,
+  // or a low result bit taken from a high source bit, give small tiles).
+  while (tile_bits(inv, a, b) < cfg.u_target) {
+    int na = std::min(a + 1, nb), nbb = std::min(b + 1, nb);
+    if ((na == a && nbb == b) || !tile_ok(inv, na, nbb, nb)) break;
+    a = na;
+    b = nbb;
+  }
+  // Tile bits: source bits [0, a) then the source bits of result bits [0, b) not already in.
+  int extra[6], ne = 0;
+  for (int p = 0; p < b; ++p)
+    if (inv[p] >= a) extra[ne++] = inv[p];  // ordered by result position
+  const int u = a + ne;
+  // t position of each tile source bit
+  int tpos[64];
+  for (int p = 0; p < 64; ++p) tpos[p] = -1;
+  for (int p = 0; p < a; ++p) tpos[p] = p;
+  for (int m = 0; m < ne; ++m) tpos[extra[m]] = a + m;
+  for (int j = 0; j < (1 << ne); ++j) {
+    uint64_t off = 0;
+    for (int m = 0; m < ne; ++m)
+      if ((j >> m) & 1) off |= 1ull << extra[m];
+    args.in_hi[j] = off;
+  }
+  // Result-order index s: bits [0, b) = result bits [0, b); above them the
+  // remaining tile bits (source bits in [0, a) whose result bit >= b), by result position.
+  int rest[6], nr = 0;
+  for (int p = b; p < nb; ++p)
+    if (inv[p] < a) {
+      if (nr == 6) return false;
+      rest[nr++] = p;  // result positions
+    }
+  if (b + nr != u) return false;
+  for (int j = 0; j < (1 << nr); ++j) {
+    uint64_t off = 0;
+    for (int m = 0; m < nr; ++m)
+      if ((j >> m) & 1) off |= 1ull << rest[m];
+    args.out_hi[j] = off;
+  }
+  int s_to_tbit[12];
+  for (int p = 0; p < b; ++p) s_to_tbit[p] = tpos[inv[p]];
+  for (int m = 0; m < nr; ++m) s_to_tbit[b + m] = tpos[inv[rest[m]]];
+  for (int j = 0; j < 64; ++j) {
+    uint32_t lo = 0, hi = 0;
+    for (int m = 0; m < 6; ++m) {
+      if (!((j >> m) & 1)) continue;
+      if (m < u) lo |= 1u << s_to_tbit[m];
+      if (m + 6 < u) hi |= 1u << s_to_tbit[m + 6];
+    }
+    args.s2t_lo[j] = (uint16_t)lo;
+    args.s2t_hi[j] = (uint16_t)hi;
+  }
+  // Outer (tile index) bits, fastest first.
+  int outer[64], no = 0;
+  for (int p = 0; p < nb; ++p)
+    if (tpos[p] < 0) outer[no++] = p;
+  if (cfg.order == 1)
+    std::stable_sort(outer, outer + no,
+                     [&](int x, int y) { return std::max(x, sigma[x]) < std::max(y, sigma[y]); });
+  for (int j = 0; j < no; ++j) {
+    args.ob_in[j] = (uint8_t)outer[j];
+    args.ob_out[j] = (uint8_t)sigma[outer[j]];
+  }
+  args.a = a;
+  args.b = b;
+  args.u = u;
+  args.n_outer = no;
+  if (u > 12 || u < 0 || no + u != nb) return false;
+  *u_out = u;
+  *no_out = no;
+  return true;
+}
+
+void launch_bits(hipStream_t stream, int num_cus, const c128 *src, c128 *dst, const PermBitsArgs &args, int u, int no,
+                 int nb, const PermTune &cfg) {
+  const uint64_t n_tiles = 1ull << no;
+  const size_t lds = (size_t)sizeof(c128) << u;
+  const uint64_t cap = (uint64_t)std::max(1, num_cus) * (uint64_t)cfg.blocks_per_cu;
+  unsigned blocks = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_tiles, cap), 1u << 30);
+  PermLaunch L{stream, blocks, lds, src, dst, n_tiles, nb <= 28, cfg.nt != 0};
+  switch (u) {
+    case 0: launch_tile<0>(L, args); break;
+    case 1: launch_tile<1>(L, args); break;
+    case 2: launch_tile<2>(L, args); break;
+    case 3: launch_tile<3>(L, args); break;
+    case 4: launch_tile<4>(L, args); break;
+    case 5: launch_tile<5>(L, args); break;
+    case 6: launch_tile<6>(L, args); break;
+    case 7: launch_tile<7>(L, args); break;
+    case 8: launch_tile<8>(L, args); break;
+    case 9: launch_tile<9>(L, args); break;
+    case 10: launch_tile<10>(L, args); break;
+    case 11: launch_tile<11>(L, args); break;
+    default: launch_tile<12>(L, args); break;
+  }
+}
+
+// Times the candidate tile shapes for one signature on the caller's buffers and
+// returns the fastest; *wrote is set when dst already holds the result.
+qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sigma, const int *inv, int nb,
+                      PermTune *best, bool *wrote) {
+  *wrote = false;
+  float best_ms = 0.0f;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t err = hipEventCreate(&e0);
+  if (err == hipSuccess) err = hipEventCreate(&e1);
+  for (const PermTune &cand : kPermCandidates) {
+    if (err != hipSuccess) break;
+    PermBitsArgs args;
+    int u, no;
+    if (!plan_bits(sigma, inv, nb, cand, args, &u, &no)) continue;
+    launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cand);  // warm-up; writes the result
+    err = hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < kPermTimedRuns && err == hipSuccess; ++r) {
+      launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cand);
+      err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (err != hipSuccess) break;
+    *wrote = true;
+    if (best_ms == 0.0f || ms < best_ms) {
+      best_ms = ms;
+      *best = cand;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return check_hip(ctx, err, "permute tuning");
+}
 }  // namespace
 
 qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
@@ -296,98 +464,37 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
     for (int i = 0; i < rank; ++i)
       for (int t = 0; t < in.width[i]; ++t) sigma[in.shift[i] + t] = out.shift[ptup[i]] + t;
     for (int p = 0; p < nb; ++p) inv[sigma[p]] = p;
+    PermTune cfg = g_perm;
+    if (g_perm_auto && n >= kPermAutoMin) {
+      std::string key((const char *)&ctx->device, sizeof(int));
+      for (int p = 0; p < nb; ++p) key.push_back((char)sigma[p]);
+      bool found = false;
+      {
+        std::lock_guard<std::mutex> lock(g_perm_mu);
+        auto it = g_perm_cache.find(key);
+        if (it != g_perm_cache.end()) {
+          cfg = it->second;
+          found = true;
+        }
+      }
+      hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      hipError_t qe = found ? hipSuccess : hipStreamIsCapturing(ctx->stream, &cap);
+      if (qe != hipSuccess) (void)hipGetLastError();  // not tuned: the query's error is not the caller's
+      if (!found && qe == hipSuccess && cap == hipStreamCaptureStatusNone) {
+        bool wrote = false;
+        qtnh_status st = tune_bits(ctx, src, dst, sigma, inv, nb, &cfg, &wrote);
+        if (st) return st;
+        std::lock_guard<std::mutex> lock(g_perm_mu);
+        if (g_perm_cache.size() >= kPermCacheMax) g_perm_cache.clear();
+        g_perm_cache[key] = cfg;
+        if (wrote) return QTNH_OK;
+      }
+    }
     PermBitsArgs args;
-    memset(&args, 0, sizeof(args));
-    int a = std::min(g_perm.a, nb), b = std::min(g_perm.b, nb);
-    if (!tile_ok(inv, a, b, nb)) a = b = std::min(6, nb);
-    // Grow the tile while it is smaller than the target (low bits kept in place,
-    // or a low result bit taken from a high source bit, give small tiles).
-    while (tile_bits(inv, a, b) < g_perm.u_target) {
-      int na = std::min(a + 1, nb), nbb = std::min(b + 1, nb);
-      if ((na == a && nbb == b) || !tile_ok(inv, na, nbb, nb)) break;
-      a = na;
-      b = nbb;
-    }
-    // Tile bits: source bits [0, a) then the source bits of result bits [0, b) not already in.
-    int extra[6], ne = 0;
-    for (int p = 0; p < b; ++p)
-      if (inv[p] >= a) extra[ne++] = inv[p];  // ordered by result position
-    const int u = a + ne;
-    // t position of each tile source bit
-    int tpos[64];
-    for (int p = 0; p < 64; ++p) tpos[p] = -1;
-    for (int p = 0; p < a; ++p) tpos[p] = p;
-    for (int m = 0; m < ne; ++m) tpos[extra[m]] = a + m;
-    for (int j = 0; j < (1 << ne); ++j) {
-      uint64_t off = 0;
-      for (int m = 0; m < ne; ++m)
-        if ((j >> m) & 1) off |= 1ull << extra[m];
-      args.in_hi[j] = off;
-    }
-    // Result-order index s: bits [0, b) = result bits [0, b); above them the
-    // remaining tile bits (source bits in [0, a) whose result bit >= b), by result position.
-    int rest[6], nr = 0;
-    for (int p = b; p < nb; ++p)
-      if (inv[p] < a) {
-        if (nr == 6) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
-        rest[nr++] = p;  // result positions
-      }
-    if (b + nr != u) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
-    for (int j = 0; j < (1 << nr); ++j) {
-      uint64_t off = 0;
-      for (int m = 0; m < nr; ++m)
-        if ((j >> m) & 1) off |= 1ull << rest[m];
-      args.out_hi[j] = off;
-    }
-    int s_to_tbit[12];
-    for (int p = 0; p < b; ++p) s_to_tbit[p] = tpos[inv[p]];
-    for (int m = 0; m < nr; ++m) s_to_tbit[b + m] = tpos[inv[rest[m]]];
-    for (int j = 0; j < 64; ++j) {
-      uint32_t lo = 0, hi = 0;
-      for (int m = 0; m < 6; ++m) {
-        if (!((j >> m) & 1)) continue;
-        if (m < u) lo |= 1u << s_to_tbit[m];
-        if (m + 6 < u) hi |= 1u << s_to_tbit[m + 6];
-      }
-      args.s2t_lo[j] = (uint16_t)lo;
-      args.s2t_hi[j] = (uint16_t)hi;
-    }
-    // Outer (tile index) bits, fastest first.
-    int outer[64], no = 0;
-    for (int p = 0; p < nb; ++p)
-      if (tpos[p] < 0) outer[no++] = p;
-    if (g_perm.order == 1)
-      std::stable_sort(outer, outer + no,
-                       [&](int x, int y) { return std::max(x, sigma[x]) < std::max(y, sigma[y]); });
-    for (int j = 0; j < no; ++j) {
-      args.ob_in[j] = (uint8_t)outer[j];
-      args.ob_out[j] = (uint8_t)sigma[outer[j]];
-    }
-    args.a = a;
-    args.b = b;
-    args.u = u;
-    args.n_outer = no;
-    if (u > 12 || u < 0 || no + u != nb) return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
-    const uint64_t n_tiles = 1ull << no;
-    const size_t lds = (size_t)sizeof(c128) << u;
-    const unsigned cap = (unsigned)std::max(1, ctx->num_cus) * (unsigned)g_perm.blocks_per_cu;
-    unsigned blocks = (unsigned)std::min<uint64_t>(n_tiles, cap);
-    PermLaunch L{ctx->stream, blocks, lds, src, dst, n_tiles, nb <= 28, g_perm.nt != 0};
-    switch (u) {
-      case 0: launch_tile<0>(L, args); break;
-      case 1: launch_tile<1>(L, args); break;
-      case 2: launch_tile<2>(L, args); break;
-      case 3: launch_tile<3>(L, args); break;
-      case 4: launch_tile<4>(L, args); break;
-      case 5: launch_tile<5>(L, args); break;
-      case 6: launch_tile<6>(L, args); break;
-      case 7: launch_tile<7>(L, args); break;
-      case 8: launch_tile<8>(L, args); break;
-      case 9: launch_tile<9>(L, args); break;
-      case 10: launch_tile<10>(L, args); break;
-      case 11: launch_tile<11>(L, args); break;
-      default: launch_tile<12>(L, args); break;
-    }
+    int u, no;
+    if (!plan_bits(sigma, inv, nb, cfg, args, &u, &no))
+      return set_error(ctx, QTNH_EINVAL, "permute: internal tile mismatch");
+    launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cfg);
     return check_hip(ctx, hipGetLastError(), "permute_bits launch");
   }
 

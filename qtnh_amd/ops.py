@@ -214,8 +214,15 @@ def scale_(x, c):
 
 
 def tune_permute(a=6, b=6, u_target=12, order=1, nt=1, blocks_per_cu=2):
-    """Tile parameters of the power-of-two permute kernel (process-wide)."""
+    """Tile parameters of the power-of-two permute kernel (process-wide);
+    turns per-permutation tuning off."""
     lib().qtnh_tune_permute(a, b, u_target, order, nt, blocks_per_cu)
+
+
+def tune_permute_auto(on=True):
+    """Per-permutation tile tuning of the power-of-two permute (on by default;
+    turning it on restores the default tile and forgets tuned shapes)."""
+    lib().qtnh_tune_permute_auto(1 if on else 0)
 
 
 def tune_zgemm(bn=0):

@@ -107,7 +107,48 @@ def test_permute_every_tile_shape():
             ops.tune_permute(a, b, u, order, nt, bpc)
             assert same_bits(host(ops.permute(dev(x), dims, p)), ref), (a, b, u, order, nt, bpc)
     finally:
-        ops.tune_permute()
+        ops.tune_permute_auto(True)
+
+
+def test_permute_tuning_keeps_bits():
+    """Per-permutation tuning gives the default tile's bits: on the tuning call
+    (candidate shapes timed into dst), on the cached call after it, and in a
+    graph captured before the signature was tuned (tuning skipped), replayed
+    with eager work and synchronisation between replays."""
+    r = 24
+    dims = [2] * r
+    rng = random.Random(2400)
+    rand = list(range(r))
+    rng.shuffle(rand)
+    low = list(range(r - 8))
+    rng.shuffle(low)
+    swap = list(range(r))
+    swap[0], swap[r - 1] = r - 1, 0
+    perms = [rand, list(range(r))[::-1], low + list(range(r - 8, r)), swap]
+    x = ops.fill_(ops.empty(1 << r), 31)
+    try:
+        ops.tune_permute_auto(False)
+        refs = [ops.permute(x, dims, p) for p in perms]
+        ops.tune_permute_auto(True)
+        for p, ref in zip(perms, refs):
+            for rep in range(2):
+                y = ops.permute(x, dims, p)
+                torch.cuda.synchronize()
+                assert torch.equal(torch.view_as_real(y), torch.view_as_real(ref)), (p, rep)
+        ops.tune_permute_auto(True)  # forget the tuned shapes
+        y = ops.empty(1 << r)
+        s = torch.cuda.Stream()
+        ops.context(stream=s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            ops.permute(x, dims, perms[0], out=y)
+        for rep in range(3):
+            ops.fill_(y, 5)
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(torch.view_as_real(y), torch.view_as_real(refs[0])), rep
+    finally:
+        ops.tune_permute_auto(True)
 
 
 @pytest.mark.parametrize("case", _perm_cases(), ids=lambda c: "%s-%s" % (c[0][:3], c[1][:3]))
