@@ -294,9 +294,27 @@ def bench_c5(torch, ops, _lib, count, rate, seed):
                      "throughput_qps": round(count / max(x["done_s"] for x in r), 2), "wall_s": round(wall, 2)}
     out["p99_speedup"] = round(out["sequential"]["p99_ms"] / out["batched"]["p99_ms"], 2)
     out["throughput_speedup"] = round(out["batched"]["throughput_qps"] / out["sequential"]["throughput_qps"], 2)
-    out["answers_bitwise_equal"] = all(
-        np.array_equal(np.array(a["value"]).view(np.uint64), np.array(b["value"]).view(np.uint64))
-        for a, b in zip(res["sequential"], res["batched"]))
+    same = lambda a, b: np.array_equal(np.array(a["value"]).view(np.uint64),  # noqa: E731
+                                       np.array(b["value"]).view(np.uint64)) and a["zsum"] == b["zsum"]
+    out["answers_bitwise_equal"] = all(same(a, b) for a, b in zip(res["sequential"], res["batched"]))
+    # Saturation: the same queries all arriving at t = 0, so throughput is the
+    # service rate of each policy rather than the arrival rate.
+    sat = [dict(q, arrival_s=0.0) for q in qs]
+    sres = {}
+    out["saturation"] = {}
+    for name, pol in (("sequential", _lib.SERVE_SEQUENTIAL), ("batched", _lib.SERVE_BATCHED)):
+        r = ops.serve(sat, budget_bytes=200 << 30, policy=pol)
+        lat = [x["latency_s"] * 1e3 for x in r]
+        sres[name] = r
+        out["saturation"][name] = {"p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
+                                   "throughput_qps": round(count / max(x["done_s"] for x in r), 2)}
+    out["saturation"]["throughput_speedup"] = round(
+        out["saturation"]["batched"]["throughput_qps"] / out["saturation"]["sequential"]["throughput_qps"], 2)
+    out["saturation"]["p99_speedup"] = round(
+        out["saturation"]["sequential"]["p99_ms"] / out["saturation"]["batched"]["p99_ms"], 2)
+    out["saturation"]["answers_bitwise_equal"] = all(
+        same(a, b) for a, b in zip(sres["sequential"], sres["batched"])) and all(
+        same(a, b) for a, b in zip(sres["batched"], res["batched"]))
     return out
 
 

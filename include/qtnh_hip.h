@@ -290,8 +290,9 @@ void qtnh_tune_stream(int max_blocks);
 void qtnh_tune_circuit(int tile_bits, int low_bits);
 /* Kernel for large states with 12-bit tiles: 2 (default) register-blocked
  * (ops applied to 16 amplitudes per thread in registers, one LDS round trip
- * per phase of ops on 4 tile bits), 1 pipelined (next tile streamed into LDS
- * during the current tile's ops), 0 plain.  Results do not depend on it. */
+ * per phase of ops on 4 tile bits), 3 register-blocked with the next tile
+ * streamed into a second LDS buffer during the current tile's phases, 1
+ * pipelined (per-op LDS passes), 0 plain.  Results do not depend on it. */
 void qtnh_tune_circuit_pipeline(int mode);
 
 #ifdef __cplusplus

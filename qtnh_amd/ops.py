@@ -256,8 +256,9 @@ def autotune_circuit(n_qubits, device=None):
 
 
 def tune_circuit_pipeline(mode=2):
-    """Fused kernel for large states: 2 register-blocked (default), 1 pipelined, 0 plain;
-    results do not depend on it."""
+    """Fused kernel for large states: 3 register-blocked with the next tile in
+    flight, 2 register-blocked (default), 1 pipelined, 0 plain; results do not
+    depend on it."""
     lib().qtnh_tune_circuit_pipeline(int(mode))
 
 

@@ -88,6 +88,50 @@ def test_runs_group_ops_by_tile_bits():
     assert small.info()[2] == 1  # a state of <= tile bits is one tile: one run
 
 
+def _fewest_runs(n, op_list, tb, low_bits):
+    """Dynamic programme over contiguous segments: fewest runs whose gate
+    qubits (with the low tile bits) fit tb bits; diagonals need no tile bit."""
+    tbe = min(tb, n)
+    low = (1 << min(low_bits, max(tbe - 3, 0), n)) - 1
+    masks = []
+    for kind, q, _ in op_list:
+        m = 0
+        if kind == 0:
+            for x in q:
+                m |= 1 << (n - 1 - x)
+        masks.append(m)
+    inf = float("inf")
+    best = [0] + [inf] * len(masks)
+    for end in range(1, len(masks) + 1):
+        m = low
+        for start in range(end - 1, -1, -1):
+            m |= masks[start]
+            if bin(m).count("1") > tbe:
+                break
+            best[end] = min(best[end], best[start] + 1)
+    return best[-1]
+
+
+@pytest.mark.parametrize("n,depth,tb", [(20, 3, 12), (24, 4, 12), (18, 3, 8), (22, 2, 13)])
+def test_runs_are_fewest_contiguous_segments(n, depth, tb):
+    """The planner's run count equals the optimum over contiguous partitions,
+    with diagonals (CZ layers, extra 3-qubit diagonals) costing no tile bits."""
+    c = ops.Circuit(n).random(depth, n + tb)
+    rng = np.random.default_rng(n)
+    for _ in range(10):
+        qs = [int(v) for v in rng.choice(n, 3, replace=False)]
+        c.diag(qs, np.exp(1j * rng.standard_normal(8)))
+    ops.tune_circuit(tb, 4)
+    try:
+        assert c.info()[2] == _fewest_runs(n, c.ops(), tb, 4)
+    finally:
+        ops.tune_circuit(12, 4)
+    only_diag = ops.Circuit(n)
+    for q in range(n - 1):
+        only_diag.diag([q, q + 1], [1, 1, 1, -1])
+    assert only_diag.info()[2] == 1
+
+
 @pytest.mark.parametrize("qubits", [[5], [0, 0], [0, 1, 2, 3], []])
 def test_invalid_gates_rejected(qubits):
     c = ops.Circuit(5)

@@ -107,11 +107,12 @@ def test_every_tile_size_bitexact(tb):
         ops.tune_circuit(12)
 
 
-@pytest.mark.parametrize("mode", [2, 1, 0])
+@pytest.mark.parametrize("mode", [3, 2, 1, 0])
 def test_large_state_kernels_bitexact(mode):
-    """22 qubits: 1024 tiles of 12 bits.  Register-blocked (2), pipelined (1)
-    and plain (0) fused kernels all equal the oracle; the circuit mixes 1-, 2-
-    and 3-qubit gates and diagonals on low and high qubits."""
+    """22 qubits: 1024 tiles of 12 bits.  Register-blocked with the next tile
+    in flight (3), register-blocked (2), pipelined (1) and plain (0) fused
+    kernels all equal the oracle; the circuit mixes 1-, 2- and 3-qubit gates
+    and diagonals on low and high qubits."""
     n = 22
     c = ops.Circuit(n).random(2, 2222)
     rng = np.random.default_rng(mode)
@@ -126,6 +127,35 @@ def test_large_state_kernels_bitexact(mode):
         c.run(s)
         assert same_bits(host(s), ref), mode
     finally:
+        ops.tune_circuit_pipeline(2)
+
+
+@pytest.mark.parametrize("mode,tb", [(3, 12), (2, 12), (1, 12), (0, 12), (0, 6)])
+def test_diagonals_outside_the_tile_bitexact(mode, tb):
+    """Diagonals on qubits outside a run's tile take the coefficient block
+    those qubits select per tile.  Random unequal coefficients (so a wrong
+    index order or outer bit shows) on 1-3 qubits mixing tile and outer ones,
+    between layers of dense gates; every large-state kernel and a 6-bit tile."""
+    n = 22
+    rng = np.random.default_rng(10 * mode + tb)
+    c = ops.Circuit(n)
+    for layer in range(2):
+        for q in range(n):
+            c.gate([q], rng.standard_normal(4) + 1j * rng.standard_normal(4))
+        for _ in range(10):
+            k = int(rng.integers(1, 4))
+            qs = [int(v) for v in rng.choice(n, k, replace=False)]
+            c.diag(qs, np.exp(1j * rng.standard_normal(1 << k)) * (1 + 0.25 * rng.standard_normal(1 << k)))
+    x = rnd_state(n, 23)
+    ref = oracle_run(c, x)
+    ops.tune_circuit(tb)
+    ops.tune_circuit_pipeline(mode)
+    try:
+        s_ = dev(x)
+        c.run(s_)
+        assert same_bits(host(s_), ref), (mode, tb)
+    finally:
+        ops.tune_circuit(12)
         ops.tune_circuit_pipeline(2)
 
 
