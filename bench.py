@@ -193,7 +193,7 @@ def bench_c3_circuit(torch, ops, _lib, n, depth, seed):
 
 def bench_c3_gemm(torch, ops, _lib, n, seed):
     """Two rank-2r tensors sharing r axes (M = N = K = n): FP64 complex GEMM vs
-    split precision (INT8 matrix cores, 6 slices); error vs the FP64 result."""
+    split precision (INT8 matrix cores, 5 slices); error vs the FP64 result."""
     r = int(round(math.log2(n)))
     a = ops.fill_(ops.empty(n * n), seed)
     b = ops.fill_(ops.empty(n * n), seed + 1)
@@ -279,7 +279,8 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
 def bench_c5(torch, ops, _lib, count, rate, seed):
     """Serving: Poisson arrivals, circuits of 10..28 qubits and depth 10..30
     (random SU(2) + CZ), half amplitude and half <Z> queries; sequential
-    one-launch-per-gate FP64 vs batched fused FP64 with resident states."""
+    one-launch-per-gate FP64 vs batched fused FP64 with resident states, at the
+    arrival rate and at saturation (every query arriving at once)."""
     import numpy as np
     qs = ops.poisson_queries(count, rate, seed, qubits=(10, 28), depth=(10, 30))
     out = {"queries": count, "rate_per_s": rate, "qubits": [10, 28], "depth": [10, 30]}
