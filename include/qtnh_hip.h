@@ -228,8 +228,9 @@ qtnh_status qtnh_net_batch_destroy(qtnh_net_batch_t b);
 #define QTNH_QUERY_AMPLITUDE 0 /* value = <basis| C |0...0> */
 #define QTNH_QUERY_EXPECT_Z 1  /* value = (<Z_0>, <Z_{n-1}>), zsum = sum_q <Z_q> */
 #define QTNH_SERVE_SEQUENTIAL 0 /* one query at a time, one launch per gate, no resident states */
-#define QTNH_SERVE_BATCHED 1    /* FIFO admission under the budget, fused batched runs by size class,
-                                   least-recently-used resident output states */
+#define QTNH_SERVE_BATCHED 1    /* continuous batching: steps of grouped fused runs over every circuit
+                                   in flight; arrivals admitted FIFO at each step while their states
+                                   fit the budget; least-recently-used resident output states */
 typedef struct {
   int n_qubits;
   int depth;
@@ -242,7 +243,7 @@ typedef struct {
   double start_s, done_s; /* seconds after the start */
   double value[2];
   double zsum;
-  int batch;  /* admission round that answered it */
+  int batch;  /* batched: step at which it was admitted; sequential: its position in service order */
   int cached; /* 1: answered from a resident state */
 } qtnh_query_result;
 /* Serves the queries in arrival order; returns when all are answered.
@@ -250,7 +251,7 @@ typedef struct {
 qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *queries, qtnh_query_result *results,
                        uint64_t budget_bytes, int policy);
 
-/* Times the fused executor at several tile shapes on a depth-2 random
+/* Times the fused executor at several tile shapes on a depth-6 random
  * circuit of min(n, 26) qubits and keeps the fastest for n-qubit circuits
  * run on this context (results do not depend on it).  Outputs may be NULL:
  * the chosen tile bits, low bits and the best time (ms). */

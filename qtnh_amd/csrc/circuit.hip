@@ -1303,6 +1303,43 @@ bool qubits_valid(int n, int k, const uint16_t *q) {
 }  // namespace
 }  // namespace qtnh_hip
 
+namespace qtnh_hip {
+
+qtnh_status circuit_run_count(qtnh_ctx_t ctx, qtnh_circuit_t c, int *nruns) {
+  qtnh_status st = compile(ctx, *c);
+  if (st) return st;
+  *nruns = (int)c->runs.size();
+  return QTNH_OK;
+}
+
+qtnh_status circuit_advance(qtnh_ctx_t ctx, int count, const qtnh_circuit_t *cs, void *const *states,
+                            const int *first_run, const int *nrun) {
+  int kmax = 0;
+  for (int i = 0; i < count; ++i) {
+    qtnh_status st = compile(ctx, *cs[i]);
+    if (st) return st;
+    if (first_run[i] < 0 || nrun[i] < 0 || (size_t)first_run[i] + (size_t)nrun[i] > cs[i]->runs.size())
+      return set_error(ctx, QTNH_EINVAL, "circuit advance: runs out of range");
+    kmax = std::max(kmax, nrun[i]);
+  }
+  std::vector<RunDesc> descs;
+  std::vector<size_t> starts;
+  for (int k = 0; k < kmax; ++k) {
+    starts.push_back(descs.size());
+    uint64_t first = 0;
+    for (int i = 0; i < count; ++i) {
+      if (k >= nrun[i]) continue;
+      RunDesc d = make_desc(*cs[i], (size_t)(first_run[i] + k), (c128 *)states[i], first);
+      first += desc_tiles(d);
+      descs.push_back(d);
+    }
+  }
+  starts.push_back(descs.size());
+  return launch_run_groups(ctx, descs, starts);
+}
+
+}  // namespace qtnh_hip
+
 using namespace qtnh_hip;
 
 extern "C" {
@@ -1508,6 +1545,7 @@ qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit
   starts.push_back(descs.size());
   return launch_run_groups(ctx, descs, starts);
 }
+
 
 qtnh_status qtnh_autotune_circuit(qtnh_ctx_t ctx, int n_qubits, int *tile_bits, int *low_bits, double *ms_out) {
   qtnh_status st = enter_ctx(ctx);

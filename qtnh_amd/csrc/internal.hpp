@@ -118,6 +118,14 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
 uint64_t split_scratch_bytes(uint64_t M, uint64_t N, uint64_t K);
 int split_slices();
 
+// Fused-circuit pieces used by the serving loop.  circuit_run_count compiles
+// the circuit for this context and returns its number of fused runs;
+// circuit_advance applies runs first_run[i] .. first_run[i] + nrun[i] - 1 of
+// circuit i to state i, the k-th of every circuit in one launch (k = 0, 1, ..).
+qtnh_status circuit_run_count(qtnh_ctx_t ctx, qtnh_circuit_t c, int *nruns);
+qtnh_status circuit_advance(qtnh_ctx_t ctx, int count, const qtnh_circuit_t *cs, void *const *states,
+                            const int *first_run, const int *nrun);
+
 // Host helpers for placement (also exported through the ABI).
 qtnh_status default_repls(int r1, int ndis1, int r2, int ndis2, int nw, const uint16_t *wires, uint16_t *repl1,
                           uint16_t *repl2);

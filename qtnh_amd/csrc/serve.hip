@@ -9,9 +9,15 @@
 // room).  Two policies:
 //  * sequential: one query at a time, gates applied one launch each (the
 //    reference's one contraction per gate), no resident states;
-//  * batched: every admitted query at once, circuits of a size class applied
-//    together with fused runs (one launch per run index for the class),
-//    smaller classes first so they are not held behind large states.
+//  * batched (continuous batching): the loop runs in steps.  Before each step
+//    it admits arrived queries while their states fit (a query whose
+//    circuit is already in flight joins it; one whose output state is
+//    resident is answered at once); the step then advances every circuit in
+//    flight by its next fused runs (up to 22 qubits to the end, 23-25 qubits
+//    by up to 8 runs, larger states by one), the k-th runs of all circuits in
+//    one launch.  Circuits that finish answer their queries and stay resident;
+//    their slots take new arrivals at the next step, while larger circuits
+//    are still in flight.
 // Both policies compute every amplitude with the same arithmetic in the same
 // order, so a query's answer does not depend on the policy or on its batch.
 #include <algorithm>
@@ -49,7 +55,6 @@ struct Resident {
   Key key;
   void *buf = nullptr;
   uint64_t bytes = 0;
-  int pins = 0;  // queries of the current batch reading it
 };
 
 struct Server {
@@ -64,12 +69,11 @@ struct Server {
       if (r.buf) (void)hipFree(r.buf);
   }
 
-  // Frees unpinned resident states (least recently used first) until `need`
-  // more bytes fit; false if they cannot.
+  // Frees resident states (least recently used first) until `need` more
+  // bytes fit; false if they cannot.
   bool make_room(uint64_t need) {
     for (auto it = lru.end(); used + need > budget && it != lru.begin();) {
       --it;
-      if (it->pins) continue;
       (void)hipFree(it->buf);
       used -= it->bytes;
       index.erase(it->key);
@@ -125,7 +129,15 @@ qtnh_status make_circuit(const Key &k, CircuitHandle &h) {
   return qtnh_circuit_random(h.c, k.depth, k.seed);
 }
 
-int size_class(int n) { return n <= 16 ? 0 : n <= 22 ? 1 : 2; }
+// A circuit in flight: its state, its next fused run and the queries waiting on it.
+struct Flight {
+  Key key{0, 0, 0};
+  std::shared_ptr<CircuitHandle> h;
+  void *buf = nullptr;
+  uint64_t bytes = 0;
+  int nruns = 0, next = 0;
+  std::vector<int> queries;
+};
 
 }  // namespace
 }  // namespace qtnh_hip
@@ -158,11 +170,22 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
   const auto t0 = std::chrono::steady_clock::now();
   size_t next = 0, done = 0;
   std::vector<int> pending;
-  int batch_id = 0;
+  int batch_id = 0;  // sequential policy: queries served so far
+  int step = 0;      // batched policy: steps run so far
+  std::vector<Flight> flight;
+  uint64_t inflight_bytes = 0;
+  struct FlightGuard {  // frees states still in flight on an error return
+    std::vector<Flight> &f;
+    ~FlightGuard() {
+      for (auto &x : f)
+        if (x.buf) (void)hipFree(x.buf);
+    }
+  } guard{flight};
   while (done < (size_t)nq) {
     double t = now_s(t0);
     while (next < order.size() && queries[order[next]].arrival_s <= t) pending.push_back(order[next++]);
-    if (pending.empty()) {
+    if (pending.empty() && flight.empty()) {
+      if (next >= order.size()) return set_error(ctx, QTNH_EINVAL, "serve: internal: no work left");
       const double wait = queries[order[next]].arrival_s - t;
       if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
       continue;
@@ -189,109 +212,110 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
       ++done;
       continue;
     }
-    // Batched: admit FIFO while the states fit; queries on a resident or
-    // already admitted circuit need no new state.
-    std::vector<int> admitted;
-    std::map<Key, int> fresh;  // key -> index into new_bufs
-    std::vector<std::pair<Key, void *>> new_bufs;
-    uint64_t need = 0;
-    size_t k = 0;
-    for (; k < pending.size(); ++k) {
-      const auto &q = queries[pending[k]];
+    // Continuous batching.  Admit waiting queries first-in first-out while
+    // their states fit: a query on a resident state is answered at once, one
+    // on a circuit already in flight joins it, a new circuit gets a state.
+    while (!pending.empty()) {
+      const int i = pending.front();
+      const auto &q = queries[i];
       const Key key{q.n_qubits, q.depth, q.seed};
       auto hit = srv.index.find(key);
       if (hit != srv.index.end()) {
-        hit->second->pins++;
-        admitted.push_back(pending[k]);
+        auto &r = results[i];
+        r.start_s = now_s(t0);
+        r.cached = 1;
+        r.batch = step;
+        st = answer(ctx, q, hit->second->buf, r);
+        if (st) return st;
+        r.done_s = now_s(t0);
+        srv.lru.splice(srv.lru.begin(), srv.lru, hit->second);
+        ++done;
+        pending.erase(pending.begin());
         continue;
       }
-      if (fresh.count(key)) {
-        admitted.push_back(pending[k]);
+      auto fl = std::find_if(flight.begin(), flight.end(), [&](const Flight &f) {
+        return f.key.n == key.n && f.key.depth == key.depth && f.key.seed == key.seed;
+      });
+      if (fl != flight.end()) {
+        fl->queries.push_back(i);
+        results[i].start_s = now_s(t0);
+        results[i].batch = step;
+        pending.erase(pending.begin());
         continue;
       }
       const uint64_t bytes = 16ull << q.n_qubits;
-      if (!srv.make_room(need + bytes)) break;
-      need += bytes;
-      fresh[key] = (int)new_bufs.size();
-      new_bufs.push_back({key, nullptr});
-      admitted.push_back(pending[k]);
-    }
-    pending.erase(pending.begin(), pending.begin() + k);
-    if (admitted.empty()) {
-      // Nothing fits beside pinned states: cannot happen (no pins outside a batch).
-      return set_error(ctx, QTNH_ENOMEM, "serve: admission stalled");
-    }
-    const double start = now_s(t0);
-    for (auto &nb : new_bufs) {
-      const uint64_t bytes = 16ull << nb.first.n;
-      st = check_hip(ctx, hipMalloc(&nb.second, bytes), "serve state alloc");
+      if (!srv.make_room(bytes + inflight_bytes)) break;  // waits for a slot to free
+      Flight f;
+      f.key = key;
+      f.bytes = bytes;
+      st = check_hip(ctx, hipMalloc(&f.buf, bytes), "serve state alloc");
       if (st) return st;
-      Resident res;
-      res.key = nb.first;
-      res.buf = nb.second;
-      res.bytes = bytes;
-      res.pins = 1;
-      srv.lru.push_front(res);
-      srv.index[nb.first] = srv.lru.begin();
-      srv.used += bytes;
-      st = init_state(ctx, nb.second, nb.first.n);
-      if (st) return st;
+      f.h = std::make_shared<CircuitHandle>();
+      st = make_circuit(key, *f.h);
+      if (!st) st = init_state(ctx, f.buf, key.n);
+      if (!st) st = circuit_run_count(ctx, f.h->c, &f.nruns);
+      if (st) {
+        (void)hipFree(f.buf);
+        return st;
+      }
+      inflight_bytes += bytes;
+      f.queries.push_back(i);
+      results[i].start_s = now_s(t0);
+      results[i].batch = step;
+      flight.push_back(std::move(f));
+      pending.erase(pending.begin());
     }
-    // Size classes, smallest first: circuits of a class in one batched run.
-    std::vector<std::vector<int>> cls(3);
-    for (size_t j = 0; j < new_bufs.size(); ++j) cls[size_class(new_bufs[j].first.n)].push_back((int)j);
-    std::vector<int> ready_class(nq, -1);
-    for (int a : admitted) {
-      const auto &q = queries[a];
-      const Key key{q.n_qubits, q.depth, q.seed};
-      ready_class[a] = fresh.count(key) ? size_class(q.n_qubits) : -1;
+    if (flight.empty()) {
+      // Nothing in flight and nothing admitted: every pending query fits an
+      // empty device (checked above), so only future arrivals remain.
+      if (!pending.empty()) return set_error(ctx, QTNH_ENOMEM, "serve: admission stalled");
+      continue;
     }
-    auto finish = [&](int cl) -> qtnh_status {
-      // Answer every admitted query whose state is ready after class `cl`.
-      for (int a : admitted) {
-        if (ready_class[a] != cl) continue;
-        const auto &q = queries[a];
-        const Key key{q.n_qubits, q.depth, q.seed};
-        auto it = srv.index[key];
-        auto &r = results[a];
-        r.start_s = start;
-        r.cached = cl < 0;
-        qtnh_status s2 = answer(ctx, q, it->buf, r);
-        if (s2) return s2;
+    // One step: every circuit in flight advances (states of up to 22 qubits
+    // to the end, 23-25 qubits by up to 8 runs, larger ones by one run: a step
+    // costs about one pass over the largest states), the k-th runs of all of
+    // them in one launch.
+    std::vector<qtnh_circuit_t> cs;
+    std::vector<void *> ss;
+    std::vector<int> first, count;
+    for (auto &f : flight) {
+      const int left = f.nruns - f.next;
+      const int adv = f.key.n <= 22 ? left : f.key.n <= 25 ? std::min(left, 8) : std::min(left, 1);
+      cs.push_back(f.h->c);
+      ss.push_back(f.buf);
+      first.push_back(f.next);
+      count.push_back(adv);
+      f.next += adv;
+    }
+    st = circuit_advance(ctx, (int)cs.size(), cs.data(), ss.data(), first.data(), count.data());
+    if (st) return st;
+    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "serve step sync");
+    if (st) return st;
+    ++step;
+    // Circuits that are done answer their queries and become resident states.
+    for (size_t j = 0; j < flight.size();) {
+      Flight &f = flight[j];
+      if (f.next < f.nruns) {
+        ++j;
+        continue;
+      }
+      for (int i : f.queries) {
+        auto &r = results[i];
+        st = answer(ctx, queries[i], f.buf, r);
+        if (st) return st;
         r.done_s = now_s(t0);
-        r.batch = batch_id;
         ++done;
       }
-      return QTNH_OK;
-    };
-    st = finish(-1);
-    if (st) return st;
-    for (int c = 0; c < 3; ++c) {
-      if (cls[c].empty()) continue;
-      std::vector<CircuitHandle> hs(cls[c].size());
-      std::vector<qtnh_circuit_t> cs;
-      std::vector<void *> ss;
-      for (size_t j = 0; j < cls[c].size(); ++j) {
-        const auto &nb = new_bufs[cls[c][j]];
-        st = make_circuit(nb.first, hs[j]);
-        if (st) return st;
-        cs.push_back(hs[j].c);
-        ss.push_back(nb.second);
-      }
-      st = qtnh_circuit_run_batch(ctx, (int)cs.size(), cs.data(), ss.data());
-      if (st) return st;
-      st = finish(c);
-      if (st) return st;
+      Resident res;
+      res.key = f.key;
+      res.buf = f.buf;
+      res.bytes = f.bytes;
+      srv.lru.push_front(res);
+      srv.index[f.key] = srv.lru.begin();
+      srv.used += f.bytes;
+      inflight_bytes -= f.bytes;
+      flight.erase(flight.begin() + (long)j);
     }
-    for (auto &r : srv.lru) r.pins = 0;
-    // Most recently used first.
-    for (int a : admitted) {
-      const auto &q = queries[a];
-      const Key key{q.n_qubits, q.depth, q.seed};
-      auto it = srv.index[key];
-      srv.lru.splice(srv.lru.begin(), srv.lru, it);
-    }
-    ++batch_id;
   }
   return QTNH_OK;
 }

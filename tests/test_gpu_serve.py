@@ -1,4 +1,5 @@
-"""Serving loop: answers are independent of policy, batch and cache.
+"""Serving loop: answers are independent of policy, batch and cache, and the
+batched policy admits arrivals while other circuits are in flight.
 
 Every answer must equal the same query computed alone (circuit applied to
 |0...0> with one launch per gate, then the amplitude or <Z>), bit for bit:
@@ -55,8 +56,8 @@ def test_resident_states_answer_repeats():
     qs = ops.poisson_queries(40, rate=0, seed=5, qubits=(4, 12), depth=(3, 6), pool=6)
     rs = ops.serve(qs, budget_bytes=1 << 30, policy=_lib.SERVE_BATCHED)
     check_answers(qs, rs)
-    # all arrive at once: one admission round, repeats of a spec share its state
-    assert len({r["batch"] for r in rs}) == 1
+    # all arrive at once and fit: all admitted at step 0, repeats of a spec share its state
+    assert {r["batch"] for r in rs} == {0}
 
 
 def test_resident_states_across_rounds():
@@ -72,6 +73,25 @@ def test_small_budget_forces_rounds_and_evictions():
     rs = ops.serve(qs, budget_bytes=budget, policy=_lib.SERVE_BATCHED)
     check_answers(qs, rs)
     assert len({r["batch"] for r in rs}) >= 4
+
+
+def test_continuous_batching_admits_while_a_large_circuit_runs():
+    """A 26-qubit circuit (about 80 fused runs, one per step) arrives first;
+    small queries arriving while it is on the GPU are admitted at later steps,
+    run beside it and finish before it; every answer equals the query alone."""
+    big = {"n_qubits": 26, "depth": 30, "seed": 77, "kind": _lib.QUERY_EXPECT_Z, "basis": 0, "arrival_s": 0.0}
+    small = ops.poisson_queries(12, rate=0, seed=21, qubits=(6, 12), depth=(4, 10))
+    for i, q in enumerate(small):
+        q["arrival_s"] = 0.001 + 0.0005 * i
+    qs = [big] + small
+    rs = ops.serve(qs, budget_bytes=2 << 30, policy=_lib.SERVE_BATCHED)
+    check_answers(qs, rs)
+    b = rs[0]
+    assert b["batch"] == 0
+    for r in rs[1:]:
+        assert r["batch"] > 0
+        assert r["start_s"] < b["done_s"]
+        assert r["done_s"] < b["done_s"]
 
 
 def test_oversized_query_rejected():
