@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain device copy of the state first (HBM counter calibration)")
     p.add_argument("--no-extra", action="store_true", help="skip the other configurations (C1, C3, C4, C5)")
-    p.add_argument("--only", default="", help="comma list of extra configurations to run (c1,c3c,c3g,c4,c5)")
+    p.add_argument("--only", default="", help="comma list of extra configurations to run (c2b,c1,c3c,c3g,c4,c5)")
     p.add_argument("--circuit-rank", type=int, default=30)
     p.add_argument("--circuit-depth", type=int, default=20)
     p.add_argument("--gemm-n", type=int, default=4096)
@@ -124,6 +124,35 @@ def measured_traffic(kernel):
 def _pct(v, q):
     import numpy as np
     return float(np.percentile(np.asarray(v), q))
+
+
+def bench_c2b(torch, ops, rank, seed):
+    """The same random 2-qubit gate as a DenseTensor contraction with the
+    default placement (the gate's output axes last in the result, an output
+    permute in the reference): one out-of-place pass, 32 N bytes."""
+    import numpy as np
+    _, gate_axes, g, _, _ = workload(rank, seed)
+    n = 1 << rank
+    x = ops.fill_(ops.empty(n), seed)
+    gt = torch.tensor(np.asarray(g, dtype=np.complex128), device="cuda")
+    out = ops.empty(n)
+    dims = [2] * rank
+    wires = [(gate_axes[0], 0), (gate_axes[1], 1)]
+    ops.contract(x, dims, gt, [2, 2, 2, 2], wires, out=out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        ops.contract(x, dims, gt, [2, 2, 2, 2], wires, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 32.0 * n / ms / 1e6
+    del x, out
+    torch.cuda.empty_cache()
+    return {"rank": rank, "gate_axes": gate_axes, "placement": "default (gate outputs last)", "ms": round(ms, 4),
+            "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "GFLOP/s": round(8.0 * n * 4 / ms / 1e6, 1)}
 
 
 def bench_c1(torch, ops, _lib):
@@ -327,6 +356,8 @@ def extra_configs(args):
     only = set(x for x in args.only.split(",") if x)
     want = lambda k: not only or k in only  # noqa: E731
     out = {}
+    if want("c2b"):
+        out["C2b_dense_gate_default_placement"] = bench_c2b(torch, ops, args.rank, args.seed)
     if want("c1"):
         out["C1_qft12_latency"] = bench_c1(torch, ops, _lib)
     if want("c3c"):

@@ -36,6 +36,16 @@ struct Closed {
   int nw;
 };
 
+// Closed tuples of a contraction whose dims are all powers of two: tuple t's
+// bit c moves the A / B offsets by a power of two, so counting t up by one
+// flips the offset bits in the prefix masks fa / fb[ctz(t + 1)].
+constexpr int kMaxClosedBits = 32;
+struct ClosedBits {
+  uint64_t fa[kMaxClosedBits];
+  uint64_t fb[kMaxClosedBits];
+  int kbits;
+};
+
 // Sum over the closed tuples in reference order (odometer, last position fastest).
 __device__ inline c128 closed_sum(const c128 *__restrict__ A, const c128 *__restrict__ B, uint64_t oa, uint64_t ob,
                                   const Closed &cl) {
@@ -99,8 +109,30 @@ struct OutBits {
 
 __global__ void __launch_bounds__(kDirThreads) contract_direct_bits(const c128 *__restrict__ A,
                                                                     const c128 *__restrict__ B, c128 *__restrict__ C,
-                                                                    OutBits out, Closed cl, uint64_t nC) {
+                                                                    OutBits out, ClosedBits cl, uint64_t nC) {
   const int lowbits = out.nbits < 8 ? out.nbits : 8;
+  __shared__ uint64_t flip_a[kMaxClosedBits], flip_b[kMaxClosedBits];
+  if ((int)threadIdx.x < cl.kbits) {
+    flip_a[threadIdx.x] = cl.fa[threadIdx.x];
+    flip_b[threadIdx.x] = cl.fb[threadIdx.x];
+  }
+  // A and B offsets of every value of each group of 8 high result bits, so
+  // that an element's offsets are a few table reads ORed together.
+  constexpr int kTabs = 7;  // 8 low + 7 x 8 high bits cover 64
+  __shared__ uint64_t tab_a[kTabs][256], tab_b[kTabs][256];
+  const int ntab = (out.nbits - lowbits + 7) / 8;
+  for (int t = 0; t < ntab; ++t) {
+    uint64_t va = 0, vb = 0;
+    for (int j = 0; j < 8; ++j) {
+      const int p = lowbits + 8 * t + j;
+      if (p >= out.nbits || !((threadIdx.x >> j) & 1u)) continue;
+      if (out.abit[p] >= 0) va |= 1ull << out.abit[p];
+      if (out.bbit[p] >= 0) vb |= 1ull << out.bbit[p];
+    }
+    tab_a[t][threadIdx.x] = va;
+    tab_b[t][threadIdx.x] = vb;
+  }
+  __syncthreads();
   uint64_t la = 0, lb = 0;
   for (int p = 0; p < lowbits; ++p) {
     uint64_t bit = (threadIdx.x >> p) & 1u;
@@ -110,13 +142,121 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_bits(const c128 *
   const uint64_t nhi = nC >> lowbits;
   if (threadIdx.x >= (1u << lowbits)) return;
   for (uint64_t h = blockIdx.x; h < nhi; h += gridDim.x) {
-    uint64_t ha = 0, hb = 0;
-    for (int p = lowbits; p < out.nbits; ++p) {
-      uint64_t bit = (h >> (p - lowbits)) & 1ull;
-      if (out.abit[p] >= 0) ha |= bit << out.abit[p];
-      if (out.bbit[p] >= 0) hb |= bit << out.bbit[p];
+    uint64_t ha = la, hb = lb;
+    for (int t = 0; t < ntab; ++t) {
+      const uint32_t v = (uint32_t)(h >> (8 * t)) & 255u;
+      ha |= tab_a[t][v];
+      hb |= tab_b[t][v];
     }
-    C[(h << lowbits) | threadIdx.x] = closed_sum(A, B, ha | la, hb | lb, cl);
+    // Terms in reference order; the loads of four terms are issued before
+    // their products are accumulated (same products, same order).
+    const uint64_t K = 1ull << cl.kbits;
+    double ar = 0.0, ai = 0.0;
+    uint64_t xa = ha, xb = hb;
+    for (uint64_t t = 0; t < K; t += 4) {
+      c128 a[4], b[4];
+      const int un = K - t < 4 ? (int)(K - t) : 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (u >= un) break;
+        a[u] = A[xa];
+        b[u] = B[xb];
+        const uint64_t tn = t + (uint64_t)u + 1;
+        if (tn < K) {
+          const int c = __builtin_ctzll(tn);
+          xa ^= flip_a[c];
+          xb ^= flip_b[c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (u >= un) break;
+        cmac(ar, ai, a[u], b[u]);
+      }
+    }
+    c128 r;
+    r.re = ar;
+    r.im = ai;
+    C[(h << lowbits) | threadIdx.x] = r;
+  }
+}
+
+// Power-of-two dims with few open B results on the lowest result bits (a
+// small gate contracted into a large state with its outputs placed last): one
+// thread per row of N = 2^NB results that share their A elements.  It loads
+// the K = 2^KB A terms of its row once and forms every result of the row from
+// them, each as 0 + sum over terms in reference order (the order and
+// arithmetic of contract_direct_bits).
+template <int NB, int KB>
+__global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *__restrict__ A,
+                                                                    const c128 *__restrict__ B, c128 *__restrict__ C,
+                                                                    OutBits out, ClosedBits cl, uint64_t nrows) {
+  constexpr int N = 1 << NB, K = 1 << KB;
+  constexpr int kTabs = 7;
+  __shared__ uint64_t tab_a[kTabs][256];
+  __shared__ c128 bco[K * N];  // bco[t N + o]: B element of term t for result o
+  const int rbits = out.nbits - NB;
+  const int ntab = (rbits + 7) / 8;
+  for (int t = 0; t < ntab; ++t) {
+    uint64_t va = 0;
+    for (int j = 0; j < 8; ++j) {
+      const int p = NB + 8 * t + j;
+      if (p < out.nbits && ((threadIdx.x >> j) & 1u)) va |= 1ull << out.abit[p];
+    }
+    tab_a[t][threadIdx.x] = va;
+  }
+  for (int i = threadIdx.x; i < K * N; i += kDirThreads) {
+    const int t = i / N, o = i % N;
+    uint64_t xb = 0;
+    for (int c = 0; c < KB; ++c)
+      if ((t >> c) & 1) xb |= cl.fb[c] ^ (c ? cl.fb[c - 1] : 0ull);
+    for (int j = 0; j < NB; ++j)
+      if ((o >> j) & 1) xb |= 1ull << out.bbit[j];
+    bco[i] = B[xb];
+  }
+  __syncthreads();
+  // A offset of term t: the single-bit masks of t's set bits.
+  uint64_t da[K];
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    uint64_t x = 0;
+#pragma unroll
+    for (int c = 0; c < KB; ++c)
+      if ((t >> c) & 1) x |= cl.fa[c] ^ (c ? cl.fa[c - 1] : 0ull);
+    da[t] = x;
+  }
+  for (uint64_t r = (uint64_t)blockIdx.x * kDirThreads + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * kDirThreads) {
+    uint64_t oa = 0;
+    for (int t = 0; t < ntab; ++t) oa |= tab_a[t][(uint32_t)(r >> (8 * t)) & 255u];
+    c128 a[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) a[t] = A[oa | da[t]];
+#pragma unroll
+    for (int o = 0; o < N; ++o) {
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int t = 0; t < K; ++t) cmac(ar, ai, a[t], bco[t * N + o]);
+      c128 v;
+      v.re = ar;
+      v.im = ai;
+      C[r * N + o] = v;
+    }
+  }
+}
+
+// Row kernels exist for NB + KB <= 5 (at most 32 coefficients, held in registers).
+constexpr int kRowsMaxBits = 5;
+template <int NB, int KB>
+void launch_rows_kb(int kb, dim3 g, hipStream_t st, const c128 *A, const c128 *B, c128 *C, const OutBits &ob,
+                    const ClosedBits &cb, uint64_t nrows) {
+  if constexpr (KB >= 1 && NB + KB <= kRowsMaxBits) {
+    if (kb == KB) {
+      hipLaunchKernelGGL((contract_direct_rows<NB, KB>), g, dim3(kDirThreads), 0, st, A, B, C, ob, cb, nrows);
+      return;
+    }
+    launch_rows_kb<NB, KB - 1>(kb, g, st, A, B, C, ob, cb, nrows);
+  } else if constexpr (KB >= 1) {
+    launch_rows_kb<NB, KB - 1>(kb, g, st, A, B, C, ob, cb, nrows);
   }
 }
 
@@ -215,10 +355,45 @@ qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const 
           ob.bbit[cbit] = (int8_t)(LB.shift[P.open2[q - n1]] + t);
       }
     }
+    // Tuple bits, lowest first: the last closed position's bits, then the one before.
+    ClosedBits cb;
+    memset(&cb, 0, sizeof(cb));
+    uint64_t fa = 0, fb = 0;
+    for (int w = P.nw - 1; w >= 0; --w) {
+      const int m = LA.width[P.cA[w]];
+      for (int j = 0; j < m; ++j) {
+        if (cb.kbits == kMaxClosedBits)
+          return set_error(ctx, QTNH_EUNSUPPORTED, "contract: more than %d closed bits", kMaxClosedBits);
+        fa |= 1ull << (LA.shift[P.cA[w]] + j);
+        fb |= 1ull << (LB.shift[P.cB[w]] + j);
+        cb.fa[cb.kbits] = fa;
+        cb.fb[cb.kbits] = fb;
+        ++cb.kbits;
+      }
+    }
+    // Few B results on the lowest result bits and few terms (2^(NB + KB) <= 32):
+    // one thread per row of B results.
+    int nb_low = 0;
+    while (nb_low < LC.nbits && ob.bbit[nb_low] >= 0) ++nb_low;
+    bool b_low_only = nb_low >= 1;
+    for (int p = nb_low; p < LC.nbits; ++p) b_low_only = b_low_only && ob.bbit[p] < 0;
+    const uint64_t cap = (uint64_t)std::max(1, ctx->num_cus) * 16;
+    if (b_low_only && cb.kbits >= 1 && nb_low + cb.kbits <= kRowsMaxBits && LC.nbits - nb_low >= 12) {
+      const uint64_t nrows = P.nC >> nb_low;
+      const dim3 g((unsigned)std::min<uint64_t>((nrows + kDirThreads - 1) / kDirThreads, cap));
+      switch (nb_low) {
+        case 1: launch_rows_kb<1, 4>(cb.kbits, g, ctx->stream, A, B, C, ob, cb, nrows); break;
+        case 2: launch_rows_kb<2, 3>(cb.kbits, g, ctx->stream, A, B, C, ob, cb, nrows); break;
+        case 3: launch_rows_kb<3, 2>(cb.kbits, g, ctx->stream, A, B, C, ob, cb, nrows); break;
+        default: launch_rows_kb<4, 1>(cb.kbits, g, ctx->stream, A, B, C, ob, cb, nrows); break;
+      }
+      return check_hip(ctx, hipGetLastError(), "contract_direct_rows launch");
+    }
     const int lowbits = std::min(LC.nbits, 8);
     const uint64_t nhi = P.nC >> lowbits;
-    unsigned blocks = (unsigned)std::min<uint64_t>(nhi, 1u << 20);
-    hipLaunchKernelGGL(contract_direct_bits, dim3(blocks), dim3(kDirThreads), 0, ctx->stream, A, B, C, ob, cl, P.nC);
+    // Blocks loop over rows of 256 results, so their offset tables are built once per block.
+    unsigned blocks = (unsigned)std::min<uint64_t>(nhi, cap);
+    hipLaunchKernelGGL(contract_direct_bits, dim3(blocks), dim3(kDirThreads), 0, ctx->stream, A, B, C, ob, cb, P.nC);
     return check_hip(ctx, hipGetLastError(), "contract_direct_bits launch");
   }
   OutGen og;

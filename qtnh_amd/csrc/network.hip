@@ -53,6 +53,9 @@ struct GDesc {
 
 __global__ void __launch_bounds__(kGThreads) grouped_direct_kernel(const GDesc *__restrict__ descs, int nd,
                                                                    uint64_t total_blocks) {
+  // Counting the closed tuple t up by one flips its bits 0 .. ctz(t + 1): the
+  // A / B offset changes by the prefix masks below (one XOR per term).
+  __shared__ uint64_t flip_a[64], flip_b[64];
   for (uint64_t blk = blockIdx.x; blk < total_blocks; blk += gridDim.x) {
     int lo = 0, hi = nd - 1;
     while (lo < hi) {
@@ -63,6 +66,17 @@ __global__ void __launch_bounds__(kGThreads) grouped_direct_kernel(const GDesc *
         hi = mid - 1;
     }
     const GDesc &D = descs[lo];
+    __syncthreads();  // the previous block's readers are done with the masks
+    if ((int)threadIdx.x < D.kbits) {
+      uint64_t fa = 0, fb = 0;
+      for (int j = 0; j <= (int)threadIdx.x; ++j) {
+        fa |= 1ull << D.ka[j];
+        fb |= 1ull << D.kb[j];
+      }
+      flip_a[threadIdx.x] = fa;
+      flip_b[threadIdx.x] = fb;
+    }
+    __syncthreads();
     const uint64_t e = (blk - D.first) * kGThreads + threadIdx.x;
     if (e >= D.nC) continue;
     uint64_t oa = 0, ob = 0;
@@ -77,18 +91,18 @@ __global__ void __launch_bounds__(kGThreads) grouped_direct_kernel(const GDesc *
     // last wire fastest (bit 0).
     double ar = 0.0, ai = 0.0;
     const uint64_t K = 1ull << D.kbits;
+    uint64_t xa = oa, xb = ob;
     for (uint64_t t = 0; t < K; ++t) {
-      uint64_t xa = oa, xb = ob;
-      for (int j = 0; j < D.kbits; ++j)
-        if ((t >> j) & 1ull) {
-          xa |= 1ull << D.ka[j];
-          xb |= 1ull << D.kb[j];
-        }
       const double2_t a = D.A[xa], b = D.B[xb];
       double pr, pi;
       cmul_parts(a.x, a.y, b.x, b.y, &pr, &pi);
       ar = ar + pr;
       ai = ai + pi;
+      if (t + 1 < K) {
+        const int c = __builtin_ctzll(t + 1);
+        xa ^= flip_a[c];
+        xb ^= flip_b[c];
+      }
     }
     D.C[e] = double2_t{ar, ai};
   }

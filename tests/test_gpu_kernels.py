@@ -287,6 +287,44 @@ def test_contract_distributed_goldens(group, nd1, nd2):
         _check_case(t1, d1, t2, d2, t3, d3, wires, r)
 
 
+# ------------------------------------------------------------------ contraction: small dense gate, outputs last
+@pytest.mark.parametrize("da,db,wires", [
+    ([2] * 16, [2, 2, 2, 2], [(15, 0), (3, 1)]),             # 2-qubit gate: 4 terms, 4 results per row
+    ([2] * 15, [2, 2], [(0, 0)]),                             # 1-qubit gate on the top axis
+    ([2] * 16, [2] * 5, [(2, 0), (9, 1), (11, 2), (4, 3)]),   # 16 terms, 2 results per row
+    ([2] * 16, [2] * 5, [(14, 0)]),                           # 2 terms, 16 results per row
+    ([2] * 16, [2] * 5, [(1, 0), (12, 1), (6, 2)]),           # 8 terms, 4 results per row
+    ([4] * 8, [4, 4], [(5, 0)]),                              # dim-4 axes: 4 terms, 4 results
+    ([4] * 7 + [2], [2, 4, 2, 2], [(7, 0), (2, 1)]),           # mixed widths: 8 terms, 4 results
+])
+def test_contract_small_gate_outputs_last_bitexact(da, db, wires):
+    """Default placement puts the gate's open axes last: the row kernel (one
+    thread per row of gate results) must equal the oracle bit for bit."""
+    a = rnd(int(np.prod(da)), 11)
+    b = rnd(int(np.prod(db)), 12)
+    out, dc, _ = ops.contract(dev(a), da, dev(b), db, wires)
+    ref, rdc = oracle.contract(a, da, b, db, wires)
+    assert dc == rdc
+    assert same_bits(host(out), ref)
+
+
+def test_contract_small_gate_custom_placement_bitexact():
+    """Gate outputs placed below the state's open axes too (not only last)."""
+    da, db = [2] * 15, [2, 2, 2, 2]
+    wires = [(4, 0), (10, 1)]
+    # state open axes keep their order at 0..12; gate outputs go to 14 and 13 (swapped)
+    open1 = [i for i in range(15) if i not in (4, 10)]
+    r1 = [X] * 15
+    for k, i in enumerate(open1):
+        r1[i] = k
+    r2 = [X, X, 14, 13]
+    a = rnd(1 << 15, 21)
+    b = rnd(16, 22)
+    out, dc, _ = ops.contract(dev(a), da, dev(b), db, wires, r1, r2)
+    ref, _ = oracle.contract(a, da, b, db, wires, r1, r2)
+    assert same_bits(host(out), ref)
+
+
 # ------------------------------------------------------------------ contraction: GEMM route
 @pytest.mark.parametrize("da,db,wires", [
     ([2] * 16, [2] * 16, [(i + 8, i) for i in range(8)]),                          # 256x256x256, no permutes
