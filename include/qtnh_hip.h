@@ -166,7 +166,13 @@ qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, 
 /* Reads op i back: kind (0 gate, 1 diagonal), k, qubits[k], coefficients. */
 qtnh_status qtnh_circuit_op(qtnh_circuit_t c, uint64_t i, int *kind, int *k, uint16_t *qubits, double *coef);
 /* Applies the circuit in place to a 2^n-amplitude state; both modes give
- * bit-identical results. */
+ * bit-identical results.  QTNH_RUN_FUSED keeps the run descriptors it uploads
+ * for each state buffer (up to 64 per circuit): a later fused run of the same
+ * circuit on the same buffer only launches kernels, so it may be captured
+ * into a HIP graph; the graph stays valid while the circuit is not modified
+ * or destroyed.  A fused run that would upload (first run on a buffer, or a
+ * circuit not yet compiled) returns QTNH_EUNSUPPORTED while the context's
+ * stream is being captured. */
 qtnh_status qtnh_circuit_run(qtnh_ctx_t ctx, qtnh_circuit_t c, void *state, int mode);
 /* Applies circuit i to states[i] (distinct buffers) for every i, fused, with
  * one launch per run index across the whole batch. */

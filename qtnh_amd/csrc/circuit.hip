@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "internal.hpp"
@@ -877,6 +878,15 @@ struct qtnh_circuit_s {
   std::vector<TileOp> tops;  // host copy of the compiled ops
   qtnh_hip::PhaseDesc *d_phases = nullptr;
   std::vector<std::pair<uint32_t, uint32_t>> run_phases;  // per run: [begin, end) into the phases
+  // Uploaded run descriptors of fused runs, per state buffer: a repeated run
+  // on the same state launches kernels only (no copy), so it can be captured
+  // into a graph.  Cleared when the circuit is recompiled.
+  struct DescCache {
+    RunDesc *d = nullptr;
+    std::vector<RunDesc> h;
+    std::vector<size_t> starts;
+  };
+  std::map<const void *, DescCache> desc_cache;
 };
 
 namespace qtnh_hip {
@@ -933,6 +943,9 @@ void plan_runs(qtnh_circuit_s &c, int tb, int low_bits) {
 }
 
 void free_compiled(qtnh_circuit_s &c) {
+  for (auto &kv : c.desc_cache)
+    if (kv.second.d) (void)hipFree(kv.second.d);
+  c.desc_cache.clear();
   if (c.d_ops) (void)hipFree(c.d_ops);
   if (c.d_coef) (void)hipFree(c.d_coef);
   if (c.d_phases) (void)hipFree(c.d_phases);
@@ -1127,15 +1140,11 @@ uint64_t desc_tiles(const RunDesc &d) { return 1ull << d.n_outer; }
 
 // Launches groups of runs: group g covers descs [starts[g], starts[g+1]), one
 // launch each, in order.  The descriptors go to the device in one copy.
-qtnh_status launch_run_groups(qtnh_ctx_t ctx, const std::vector<RunDesc> &descs, const std::vector<size_t> &starts) {
-  if (descs.empty()) return QTNH_OK;
-  const uint64_t bytes = descs.size() * sizeof(RunDesc);
-  qtnh_status st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, bytes);
-  if (st) return st;
-  st = check_hip(ctx, hipMemcpyAsync(ctx->desc, descs.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
-                 "run descriptor upload");
-  if (st) return st;
-  const RunDesc *dd = (const RunDesc *)ctx->desc;
+// Launches the run groups [starts[g], starts[g+1]) of descriptors already on
+// the device (dd); descs is their host copy.
+qtnh_status launch_groups(qtnh_ctx_t ctx, const RunDesc *dd, const std::vector<RunDesc> &descs,
+                          const std::vector<size_t> &starts) {
+  qtnh_status st = QTNH_OK;
   for (size_t g = 0; g + 1 < starts.size(); ++g) {
     const size_t b = starts[g], e = starts[g + 1];
     if (b == e) continue;
@@ -1194,17 +1203,55 @@ qtnh_status launch_run_groups(qtnh_ctx_t ctx, const std::vector<RunDesc> &descs,
   return QTNH_OK;
 }
 
+// Uploads the descriptors into the context's descriptor buffer, then launches.
+qtnh_status launch_run_groups(qtnh_ctx_t ctx, const std::vector<RunDesc> &descs, const std::vector<size_t> &starts) {
+  if (descs.empty()) return QTNH_OK;
+  const uint64_t bytes = descs.size() * sizeof(RunDesc);
+  qtnh_status st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, bytes);
+  if (st) return st;
+  st = check_hip(ctx, hipMemcpyAsync(ctx->desc, descs.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+                 "run descriptor upload");
+  if (st) return st;
+  return launch_groups(ctx, (const RunDesc *)ctx->desc, descs, starts);
+}
+
+constexpr size_t kDescCacheMax = 64;  // states per circuit with cached descriptors
+
 qtnh_status run_fused(qtnh_ctx_t ctx, qtnh_circuit_s &c, c128 *psi) {
   qtnh_status st = compile(ctx, c);
   if (st) return st;
-  std::vector<RunDesc> descs;
-  std::vector<size_t> starts;
-  for (size_t ri = 0; ri < c.runs.size(); ++ri) {
-    starts.push_back(descs.size());
-    descs.push_back(make_desc(c, ri, psi, 0));
+  auto hit = c.desc_cache.find(psi);
+  if (hit != c.desc_cache.end()) return launch_groups(ctx, hit->second.d, hit->second.h, hit->second.starts);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess) {
+    (void)hipGetLastError();
+    cap = hipStreamCaptureStatusNone;
   }
-  starts.push_back(descs.size());
-  return launch_run_groups(ctx, descs, starts);
+  if (cap != hipStreamCaptureStatusNone)
+    return set_error(ctx, QTNH_EUNSUPPORTED,
+                     "circuit_run: first fused run of this circuit on this state during graph capture "
+                     "(run it once on the state before capturing)");
+  qtnh_circuit_s::DescCache dc;
+  for (size_t ri = 0; ri < c.runs.size(); ++ri) {
+    dc.starts.push_back(dc.h.size());
+    dc.h.push_back(make_desc(c, ri, psi, 0));
+  }
+  dc.starts.push_back(dc.h.size());
+  if (dc.h.empty()) return QTNH_OK;
+  if (c.desc_cache.size() >= kDescCacheMax) return launch_run_groups(ctx, dc.h, dc.starts);
+  const uint64_t bytes = dc.h.size() * sizeof(RunDesc);
+  st = check_hip(ctx, hipMalloc((void **)&dc.d, bytes), "run descriptor alloc");
+  if (st) return st;
+  auto &e = c.desc_cache[psi];
+  e = std::move(dc);
+  st = check_hip(ctx, hipMemcpyAsync(e.d, e.h.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+                 "run descriptor upload");
+  if (st) {
+    (void)hipFree(e.d);
+    c.desc_cache.erase(psi);
+    return st;
+  }
+  return launch_groups(ctx, e.d, e.h, e.starts);
 }
 
 qtnh_status run_per_gate(qtnh_ctx_t ctx, const qtnh_circuit_s &c, c128 *psi) {

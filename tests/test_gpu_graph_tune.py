@@ -1,11 +1,15 @@
-"""Prepared network batches (optionally replayed as one HIP graph) and the
-circuit tile autotuner: results must not depend on either.
+"""Prepared network batches (optionally replayed as one HIP graph), fused
+circuit runs captured into graphs, and the circuit tile autotuner: results
+must not depend on any of them.
 
   * a prepared batch, eager or graph-replayed, equals the one-at-a-time
     executor bitwise, on every replay, with unrelated eager GPU work and
     synchronisation between replays;
   * new leaf values (set_leaf) give the same bits as a network built with
     those values from the start;
+  * a fused circuit run captured into a HIP graph replays bitwise like
+    eager runs; a run that would need a descriptor upload is refused during
+    capture;
   * after autotuning, fused circuits give the same bits as before.
 """
 import numpy as np
@@ -86,6 +90,54 @@ def test_set_leaf_equals_network_built_with_the_values(use_graph):
             assert same_bits(w, g), (use_graph, rep)
         torch.cuda.synchronize()
     batch.close()
+
+
+@pytest.mark.parametrize("n", [12, 20])
+def test_fused_circuit_run_captured_in_a_graph(n):
+    """After one eager run on a state, a fused run of the same circuit on that
+    state launches kernels only: captured into a HIP graph and replayed (with
+    eager work in between) it equals the eager runs bitwise, replay after
+    replay."""
+    c = ops.Circuit(n).random(4, 31 + n)
+    x0 = ops.fill_(ops.empty(1 << n), 8)
+    ref = x0.clone()
+    snaps = []
+    for _ in range(4):
+        c.run(ref)
+        snaps.append(ref.clone())
+    torch.cuda.synchronize()
+    s = x0.clone()
+    cs = torch.cuda.Stream()
+    with torch.cuda.stream(cs):
+        c.run(s)  # eager: uploads the run descriptors for this state
+    cs.synchronize()
+    assert torch.equal(torch.view_as_real(s), torch.view_as_real(snaps[0]))
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=cs):
+        c.run(s)
+    other = ops.fill_(ops.empty(1 << 16), 3)
+    for k in range(1, 4):
+        g.replay()
+        ops.scale_(other, 0.5)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.view_as_real(s), torch.view_as_real(snaps[k])), k
+
+
+def test_first_fused_run_inside_capture_is_refused():
+    """A fused run on a state the circuit has not run on would have to upload
+    descriptors: during capture it is refused before touching the stream."""
+    c = ops.Circuit(10).random(2, 3)
+    t = ops.fill_(ops.empty(1 << 10), 2)
+    c.run(t)  # compiled, descriptors cached for t (kept alive: s gets another address)
+    s = ops.fill_(ops.empty(1 << 10), 1)
+    assert s.data_ptr() != t.data_ptr()
+    cs = torch.cuda.Stream()
+    ops.context(stream=cs)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(_lib.QtnhError):
+        with torch.cuda.graph(g, stream=cs):
+            c.run(s)
 
 
 def test_autotune_keeps_bits():
