@@ -442,13 +442,15 @@ ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t 
   R.ws_bytes = 0;
   if (!R.direct) {
     // Same layout as launch_contract below: permuted A, permuted B, natural C,
-    // then split scratch or split-K partials.
+    // then split scratch, split-K partials or GEMM depth-chunk partials.
     auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
     R.ws_bytes = pad(P.M * P.K) + pad(P.K * P.N) + pad(P.nC);
     if (use_splitk(P))
       R.ws_bytes += pad(P.M * P.N * splitk_chunks(P.M * P.N, P.K));
     else if (precision == QTNH_PREC_SPLIT)
       R.ws_bytes += (split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull;
+    else if (zgemm_chunks(P.M, P.N, P.K) > 1)
+      R.ws_bytes += pad(P.M * P.N * zgemm_chunks(P.M, P.N, P.K));
   }
   return R;
 }
@@ -490,9 +492,11 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
   const uint64_t nA = P.M * P.K, nB = P.K * P.N;
   auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
   const uint64_t kchunks = splitk ? splitk_chunks(P.M * P.N, P.K) : 0;
-  const uint64_t sbytes = split    ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull)
-                          : splitk ? pad(P.M * P.N * kchunks)
-                                   : 0;
+  const uint64_t zchunks = (split || splitk) ? 1 : zgemm_chunks(P.M, P.N, P.K);
+  const uint64_t sbytes = split         ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull)
+                          : splitk      ? pad(P.M * P.N * kchunks)
+                          : zchunks > 1 ? pad(P.M * P.N * zchunks)
+                                        : 0;
   const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC)) + sbytes;
   char *ws = nullptr;
   if (bytes && ext_ws) {
@@ -534,7 +538,7 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
   } else if (split) {
     st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + off, sbytes, err_out);
   } else {
-    st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N);
+    st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N, zchunks > 1 ? (c128 *)(ws + off) : nullptr);
     if (err_out) *err_out = 0.0;
   }
   if (st) return st;

@@ -103,7 +103,10 @@ qtnh_status launch_zero(qtnh_ctx_t ctx, void *p, uint64_t bytes);
 
 // Complex GEMM C[MxN] = A[MxK] * B[KxN], row-major with leading dims lda/ldb/ldc (elements).
 qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N,
-                             uint64_t K, uint64_t lda, uint64_t ldb, uint64_t ldc);
+                             uint64_t K, uint64_t lda, uint64_t ldb, uint64_t ldc, c128 *partials);
+// Depth chunks launch_zgemm_f64 uses for this shape (1: none); with more than
+// one it needs `partials` of chunks * M * N elements.
+uint64_t zgemm_chunks(uint64_t M, uint64_t N, uint64_t K);
 
 // C[MxN] = A[MxK] * Bt[NxK]^T for few outputs: each sum in `chunks` ordered
 // chunks over separate workgroups (partials: M*N*chunks elements), then summed

@@ -34,10 +34,13 @@ struct ZTile {
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small");
 };
 
+// blockIdx.y selects a chunk of the depth: chunk c sums k in [c kc, (c + 1) kc)
+// and writes its tile to C + c cstride (one chunk: the whole depth, into C).
 template <int BM, int BN>
 __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restrict__ A, const c128 *__restrict__ B,
                                                              c128 *__restrict__ C, uint64_t M, uint64_t N, uint64_t K,
-                                                             uint64_t lda, uint64_t ldb, uint64_t ldc) {
+                                                             uint64_t lda, uint64_t ldb, uint64_t ldc, uint64_t kc,
+                                                             uint64_t cstride) {
   using Z = ZTile<BM, BN>;
   __shared__ double2_t As[BM * Z::LDA];
   __shared__ double2_t Bs[BK * Z::LDB];
@@ -49,6 +52,9 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
   const uint64_t ntn = (N + BN - 1) / BN;
   const uint64_t m0 = ((uint64_t)blockIdx.x / ntn) * BM;
   const uint64_t n0 = ((uint64_t)blockIdx.x % ntn) * BN;
+  const uint64_t kb = (uint64_t)blockIdx.y * kc;
+  const uint64_t ke = kb + kc < K ? kb + kc : K;
+  C += (uint64_t)blockIdx.y * cstride;
 
   double4_t accR[2][Z::TN], accI[2][Z::TN];
 #pragma unroll
@@ -69,18 +75,18 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
       const int idx = tid + r * kThreads;
       const int ar = idx / BK, ac = idx % BK;
       const uint64_t gm = m0 + ar, gk = k0 + ac;
-      pa[r] = (gm < M && gk < K) ? A2[gm * lda + gk] : zero;
+      pa[r] = (gm < M && gk < ke) ? A2[gm * lda + gk] : zero;
     }
 #pragma unroll
     for (int r = 0; r < Z::B_PER; ++r) {
       const int idx = tid + r * kThreads;
       const int br = idx / BN, bc = idx % BN;
       const uint64_t gk = k0 + br, gn = n0 + bc;
-      pb[r] = (gk < K && gn < N) ? B2[gk * ldb + gn] : zero;
+      pb[r] = (gk < ke && gn < N) ? B2[gk * ldb + gn] : zero;
     }
   };
-  fetch(0);
-  for (uint64_t k0 = 0; k0 < K; k0 += BK) {
+  fetch(kb);
+  for (uint64_t k0 = kb; k0 < ke; k0 += BK) {
 #pragma unroll
     for (int r = 0; r < Z::A_PER; ++r) {
       const int idx = tid + r * kThreads;
@@ -92,7 +98,7 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
       Bs[(idx / BN) * Z::LDB + idx % BN] = pb[r];
     }
     __syncthreads();
-    if (k0 + BK < K) fetch(k0 + BK);  // in flight during the MFMAs below
+    if (k0 + BK < ke) fetch(k0 + BK);  // in flight during the MFMAs below
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kl = kk + (lane >> 4);
@@ -202,20 +208,67 @@ static int g_zgemm_bn = 0;
 
 extern "C" void qtnh_tune_zgemm(int bn) { g_zgemm_bn = (bn == 64 || bn == 128) ? bn : 0; }
 
-qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N, uint64_t K,
-                             uint64_t lda, uint64_t ldb, uint64_t ldc) {
+// Depth chunks of a GEMM with few output tiles: enough chunks that tiles x
+// chunks reaches about kZgemmTargetBlocks workgroups, each summing at least
+// kZgemmMinChunk terms.  Depends on the shape only (the network planner sizes
+// workspaces with it), so every caller of a shape runs the same chunks.
+constexpr uint64_t kZgemmTargetBlocks = 512, kZgemmMinChunk = 512;
+
+static int zgemm_bn(uint64_t M, uint64_t N) {
   int bn = g_zgemm_bn;
   if (bn == 0) bn = (((M + 63) / 64) * ((N + 63) / 64) < 2048) ? 128 : 64;
   if (N < 128) bn = 64;
+  return bn;
+}
+
+uint64_t zgemm_chunks(uint64_t M, uint64_t N, uint64_t K) {
+  const int bn = zgemm_bn(M, N);
+  const uint64_t tiles = ((N + bn - 1) / bn) * ((M + 63) / 64);
+  if (tiles == 0 || tiles * 2 > kZgemmTargetBlocks || K < 2 * kZgemmMinChunk) return 1;
+  const uint64_t want = (kZgemmTargetBlocks + tiles - 1) / tiles;
+  return std::max<uint64_t>(1, std::min<uint64_t>(want, K / kZgemmMinChunk));
+}
+
+// C = sum over chunks of part[c] (M x N each), in chunk order.
+__global__ void zgemm_chunk_sum_kernel(const double2_t *__restrict__ part, double2_t *__restrict__ C, uint64_t M,
+                                       uint64_t N, uint64_t ldc, uint64_t chunks) {
+  const uint64_t MN = M * N;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < MN; i += (uint64_t)gridDim.x * blockDim.x) {
+    double2_t s = part[i];
+    for (uint64_t c = 1; c < chunks; ++c) {
+      const double2_t v = part[c * MN + i];
+      s.x = s.x + v.x;
+      s.y = s.y + v.y;
+    }
+    C[(i / N) * ldc + i % N] = s;
+  }
+}
+
+qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 *C, uint64_t M, uint64_t N, uint64_t K,
+                             uint64_t lda, uint64_t ldb, uint64_t ldc, c128 *partials) {
+  const int bn = zgemm_bn(M, N);
   const uint64_t tiles = ((N + bn - 1) / bn) * ((M + 63) / 64);
   if (tiles > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "zgemm: grid too large");
+  const uint64_t chunks = zgemm_chunks(M, N, K);
+  if (chunks > 1 && !partials) return set_error(ctx, QTNH_EINVAL, "zgemm: depth chunks need a partial buffer");
+  const uint64_t kc = chunks > 1 ? (K + chunks - 1) / chunks : K;
+  const uint64_t kcr = chunks > 1 ? (kc + BK - 1) / BK * BK : K;  // chunk bounds on BK steps
+  const uint64_t nchunks = chunks > 1 ? (K + kcr - 1) / kcr : 1;
+  c128 *out = chunks > 1 ? partials : C;
+  const uint64_t out_ld = chunks > 1 ? N : ldc;
+  const uint64_t cstride = chunks > 1 ? M * N : 0;
+  const dim3 grid((unsigned)tiles, (unsigned)nchunks);
   if (bn == 128)
-    hipLaunchKernelGGL((zgemm_f64_kernel<64, 128>), dim3((unsigned)tiles), dim3(kThreads), 0, ctx->stream, A, B, C, M,
-                       N, K, lda, ldb, ldc);
+    hipLaunchKernelGGL((zgemm_f64_kernel<64, 128>), grid, dim3(kThreads), 0, ctx->stream, A, B, out, M, N, K, lda, ldb,
+                       out_ld, kcr, cstride);
   else
-    hipLaunchKernelGGL((zgemm_f64_kernel<64, 64>), dim3((unsigned)tiles), dim3(kThreads), 0, ctx->stream, A, B, C, M, N,
-                       K, lda, ldb, ldc);
-  return check_hip(ctx, hipGetLastError(), "zgemm launch");
+    hipLaunchKernelGGL((zgemm_f64_kernel<64, 64>), grid, dim3(kThreads), 0, ctx->stream, A, B, out, M, N, K, lda, ldb,
+                       out_ld, kcr, cstride);
+  qtnh_status st = check_hip(ctx, hipGetLastError(), "zgemm launch");
+  if (st || nchunks <= 1) return st;
+  hipLaunchKernelGGL(zgemm_chunk_sum_kernel, dim3(grid_for(M * N, 256, 65536)), dim3(256), 0, ctx->stream,
+                     (const double2_t *)partials, (double2_t *)C, M, N, ldc, nchunks);
+  return check_hip(ctx, hipGetLastError(), "zgemm chunk sum launch");
 }
 
 }  // namespace qtnh_hip

@@ -123,6 +123,7 @@ def test_fused_circuit_run_captured_in_a_graph(n):
         assert torch.equal(torch.view_as_real(s), torch.view_as_real(snaps[k])), k
 
 
+@pytest.mark.filterwarnings("ignore:The CUDA Graph is empty")
 def test_first_fused_run_inside_capture_is_refused():
     """A fused run on a state the circuit has not run on would have to upload
     descriptors: during capture it is refused before touching the stream."""

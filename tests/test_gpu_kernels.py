@@ -331,6 +331,8 @@ def test_contract_small_gate_custom_placement_bitexact():
     ([2] * 16, [2] * 16, [(0, 15), (3, 2), (9, 7), (12, 0), (5, 9), (14, 11), (7, 4), (1, 13)]),
     ([3, 5, 7, 4, 9], [4, 9, 6, 11], [(3, 0), (4, 1)]),                             # odd sizes, tile edges
     ([8, 16, 32], [32, 16, 40], [(2, 0), (1, 1)]),
+    ([2] * 20, [2] * 21, [(7 + i, i) for i in range(13)]),                        # 128x256x8192: 16 depth chunks
+    ([2] * 20, [2] * 20, [(8 + i, 11 - i) for i in range(12)]),                   # 256x256x4096: 8 chunks, permuted B
 ])
 def test_contract_gemm_route(da, db, wires):
     a = rnd(int(np.prod(da)), 1)
