@@ -212,7 +212,7 @@ extern "C" void qtnh_tune_zgemm(int bn) { g_zgemm_bn = (bn == 64 || bn == 128) ?
 // chunks reaches about kZgemmTargetBlocks workgroups, each summing at least
 // kZgemmMinChunk terms.  Depends on the shape only (the network planner sizes
 // workspaces with it), so every caller of a shape runs the same chunks.
-constexpr uint64_t kZgemmTargetBlocks = 512, kZgemmMinChunk = 512;
+constexpr uint64_t kZgemmTargetBlocks = 1024, kZgemmMinChunk = 256;
 
 static int zgemm_bn(uint64_t M, uint64_t N) {
   int bn = g_zgemm_bn;
