@@ -162,48 +162,61 @@ template <int KB, int KH>
 __global__ void __launch_bounds__(kGateThreads) gate_lanes_kernel(c128 *__restrict__ psi, LaneGateArgs<KB, KH> args,
                                                                   uint64_t n_rows) {
   constexpr int K = 1 << KB, H = 1 << KH;
+  // Rows per thread per step: their loads are all issued before the first
+  // shuffle, so a wave keeps several lines in flight.
+  constexpr int U = KH >= 2 ? 1 : 4 >> KH;
   double2_t *p2 = reinterpret_cast<double2_t *>(psi);
   const int lane = threadIdx.x & 63;
   const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
-  // Every lane of a wave takes part in the shuffles: the loop bound is wave-uniform
-  // (n_rows is a multiple of 64 and so is the stride).
-  for (uint64_t t = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; t < n_rows; t += stride) {
-    uint64_t e = t;
+  // Every lane of a wave takes part in the shuffles: the loop bounds are
+  // wave-uniform (n_rows is a multiple of 64 and so is the stride).
+  for (uint64_t t0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; t0 < n_rows; t0 += stride * U) {
+    double2_t x[U][H];
+    uint64_t e[U];
 #pragma unroll
-    for (int j = 0; j < KH; ++j) {
-      const int p = args.hzpos[j];
-      e = ((e >> p) << (p + 1)) | (e & ((1ull << p) - 1));
-    }
-    double2_t x[H];
+    for (int u = 0; u < U; ++u) {
+      const uint64_t t = t0 + (uint64_t)u * stride;
+      if (t >= n_rows) break;  // wave-uniform
+      e[u] = t;
 #pragma unroll
-    for (int h = 0; h < H; ++h) x[h] = __builtin_nontemporal_load(p2 + e + args.hoff[h]);
-    // All K inputs of this lane's groups: input i sits in lane (lane & ~mask) | lbits(i), slot h(i).
-    double2_t in[K];
-#pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const int src = (lane & ~(int)args.lane_mask) | args.src_lbits[i];
-      const double2_t v = x[args.src_h[i]];
-      in[i].x = __shfl(v.x, src, 64);
-      in[i].y = __shfl(v.y, src, 64);
-    }
-#pragma unroll
-    for (int h = 0; h < H; ++h) {
-      // This lane's output index for high combination h.
-      int o = 0;
-#pragma unroll
-      for (int j = 0; j < KB; ++j) {
-        const int bit = args.jh[j] >= 0 ? ((h >> args.jh[j]) & 1) : ((lane >> args.jpos[j]) & 1);
-        o |= bit << (KB - 1 - j);
+      for (int j = 0; j < KH; ++j) {
+        const int p = args.hzpos[j];
+        e[u] = ((e[u] >> p) << (p + 1)) | (e[u] & ((1ull << p) - 1));
       }
-      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int h = 0; h < H; ++h) x[u][h] = __builtin_nontemporal_load(p2 + e[u] + args.hoff[h]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (t0 + (uint64_t)u * stride >= n_rows) break;  // wave-uniform
+      // All K inputs of this lane's groups: input i sits in lane (lane & ~mask) | lbits(i), slot h(i).
+      double2_t in[K];
 #pragma unroll
       for (int i = 0; i < K; ++i) {
-        const c128 xi = {in[i].x, in[i].y};
-        const c128 pr = cmul(xi, args.g[i * K + o]);
-        ar = ar + pr.re;
-        ai = ai + pr.im;
+        const int src = (lane & ~(int)args.lane_mask) | args.src_lbits[i];
+        const double2_t v = x[u][args.src_h[i]];
+        in[i].x = __shfl(v.x, src, 64);
+        in[i].y = __shfl(v.y, src, 64);
       }
-      __builtin_nontemporal_store(double2_t{ar, ai}, p2 + e + args.hoff[h]);
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        // This lane's output index for high combination h.
+        int o = 0;
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          const int bit = args.jh[j] >= 0 ? ((h >> args.jh[j]) & 1) : ((lane >> args.jpos[j]) & 1);
+          o |= bit << (KB - 1 - j);
+        }
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const c128 xi = {in[i].x, in[i].y};
+          const c128 pr = cmul(xi, args.g[i * K + o]);
+          ar = ar + pr.re;
+          ai = ai + pr.im;
+        }
+        __builtin_nontemporal_store(double2_t{ar, ai}, p2 + e[u] + args.hoff[h]);
+      }
     }
   }
 }
