@@ -62,15 +62,26 @@ def workload(rank, seed):
     return ptup, gate_axes, g, diag_axes, d
 
 
-def cpu_baseline(args, ptup_full):
-    """The oracle (single-threaded C restatement of the reference path) on a
-    rank-`cpu_rank` sample of the same three kernels, scaled per element to
-    the full rank."""
+def _host_threads():
+    """Threads for the threaded CPU baseline: the CPUs this process may use,
+    at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(args, ptup_full, threads=1, rank=None):
+    """The oracle's restatement of the reference path (one thread: the C
+    oracle itself; several: the same per-element arithmetic split over POSIX
+    threads) on a rank-`rank` sample of the same three kernels, scaled per
+    element to the full rank."""
     import numpy as np
 
     import oracle
 
-    r = args.cpu_rank
+    r = rank or args.cpu_rank
     n = 1 << r
     rng = random.Random(args.seed + 1)
     ptup = list(range(r))
@@ -79,23 +90,31 @@ def cpu_baseline(args, ptup_full):
     x = oracle.fill(n, args.seed, 1)
     dims = [2] * r
     t0 = time.perf_counter()
-    y = oracle.permute(x, dims, ptup)
-    t1 = time.perf_counter()
-    y = oracle.gate_apply(y, dims, gate_axes, np.asarray(g))
-    t2 = time.perf_counter()
-    y = oracle.diag_apply(y, dims, diag_axes, np.asarray(d))
+    if threads == 1:
+        y = oracle.permute(x, dims, ptup)
+        t1 = time.perf_counter()
+        y = oracle.gate_apply(y, dims, gate_axes, np.asarray(g), inplace=True)
+        t2 = time.perf_counter()
+        y = oracle.diag_apply(y, dims, diag_axes, np.asarray(d), inplace=True)
+    else:
+        y = oracle.permute_mt(x, dims, ptup, threads)
+        t1 = time.perf_counter()
+        y = oracle.gate_apply_mt(y, dims, gate_axes, np.asarray(g), threads, inplace=True)
+        t2 = time.perf_counter()
+        y = oracle.diag_apply_mt(y, dims, diag_axes, np.asarray(d), threads, inplace=True)
     t3 = time.perf_counter()
     scale = float(1 << args.rank) / n
     t_step = (t3 - t0) * scale
     N = float(1 << args.rank)
     flops = 8 * N * 4 + 6 * N
+    how = "the C oracle, one thread" if threads == 1 else "the oracle's arithmetic on %d POSIX threads" % threads
     return {
         "value": flops / t_step / 1e9,
         "unit": "GFLOP/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
         "sample": "rank-%d state (2^%d elements), one pass each of permute, 2-qubit gate, diagonal phase "
-                  "through the oracle; per-element times scaled x%d to rank %d" % (r, r, int(scale), args.rank),
+                  "through %s; per-element times scaled x%d to rank %d" % (r, r, how, int(scale), args.rank),
         "ms_per_step": t_step * 1e3,
         "ms_sample": {"permute": (t1 - t0) * 1e3, "gate": (t2 - t1) * 1e3, "diag": (t3 - t2) * 1e3},
         "host_cpu": _cpu_model(),
@@ -480,7 +499,9 @@ def main():
     if not args.no_extra:
         out["configs"] = extra_configs(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, ptup)
+        threads = _host_threads()
+        out["cpu_baseline"] = cpu_baseline(args, ptup, threads=threads, rank=args.cpu_rank + 2 if threads > 1 else None)
+        out["cpu_baseline_1core"] = cpu_baseline(args, ptup)
     print(json.dumps(out))
 
 

@@ -47,6 +47,9 @@ def _load():
     lib.or_gate_apply.argtypes = [_dp, ctypes.c_int, _u64p, ctypes.c_int, _u16p, _dp]
     lib.or_diag_apply.argtypes = [_dp, ctypes.c_int, _u64p, ctypes.c_int, _u16p, _dp]
     lib.or_qft.argtypes = [_dp, ctypes.c_int, _dp]
+    lib.or_permute_mt.argtypes = [_dp, _dp, ctypes.c_int, _u64p, _u16p, ctypes.c_int]
+    lib.or_gate_apply_mt.argtypes = [_dp, ctypes.c_int, _u64p, ctypes.c_int, _u16p, _dp, ctypes.c_int]
+    lib.or_diag_apply_mt.argtypes = [_dp, ctypes.c_int, _u64p, ctypes.c_int, _u16p, _dp, ctypes.c_int]
     lib.or_norm2.restype = ctypes.c_double
     lib.or_norm2.argtypes = [_dp, ctypes.c_uint64]
     lib.or_multiset_hash.restype = ctypes.c_uint64
@@ -191,6 +194,38 @@ def diag_apply(x, dims, axes, dvals, inplace=False):
     a, ap = _u16(axes)
     dv, dvp = _c128(dvals)
     _check(lib().or_diag_apply(x.ctypes.data_as(_dp), len(dims), dp, len(axes), ap, dvp), "diag_apply")
+    return x
+
+
+# ---- CPU timing baseline: the same arithmetic split over threads (not checkers).
+def permute_mt(x, dims, ptup, threads):
+    x, xp = _c128(x)
+    out = np.empty_like(x)
+    d, dp = _u64(dims)
+    p, pp = _u16(ptup)
+    _check(lib().or_permute_mt(xp, out.ctypes.data_as(_dp), len(dims), dp, pp, int(threads)), "permute_mt")
+    return out
+
+
+def gate_apply_mt(x, dims, axes, g, threads, inplace=False):
+    if not inplace:
+        x = np.array(x, dtype=np.complex128, copy=True)
+    d, dp = _u64(dims)
+    a, ap = _u16(axes)
+    g, gp = _c128(g)
+    _check(lib().or_gate_apply_mt(x.ctypes.data_as(_dp), len(dims), dp, len(axes), ap, gp, int(threads)),
+           "gate_apply_mt")
+    return x
+
+
+def diag_apply_mt(x, dims, axes, dvals, threads, inplace=False):
+    if not inplace:
+        x = np.array(x, dtype=np.complex128, copy=True)
+    d, dp = _u64(dims)
+    a, ap = _u16(axes)
+    dv, dvp = _c128(dvals)
+    _check(lib().or_diag_apply_mt(x.ctypes.data_as(_dp), len(dims), dp, len(axes), ap, dvp, int(threads)),
+           "diag_apply_mt")
     return x
 
 

@@ -403,3 +403,227 @@ uint64_t or_multiset_hash(const double *x, uint64_t n) {
   }
   return h;
 }
+
+/* ---------------------------------------------------------------------------
+ * Multi-threaded restatements for the CPU timing baseline (not checkers).
+ * Each thread takes a contiguous range of elements (permute, diagonal) or of
+ * gate groups and does exactly the per-element arithmetic of the functions
+ * above, so the results are equal to theirs bit for bit.
+ * --------------------------------------------------------------------------- */
+#include <pthread.h>
+
+#define MAXTHREADS 256
+typedef void (*range_fn)(void *ctx, uint64_t begin, uint64_t end);
+struct range_job {
+  range_fn fn;
+  void *ctx;
+  uint64_t b, e;
+};
+
+static void *range_thread(void *p) {
+  struct range_job *j = (struct range_job *)p;
+  j->fn(j->ctx, j->b, j->e);
+  return NULL;
+}
+
+/* Splits [0, n) into nthreads contiguous ranges; a thread that cannot be
+ * started runs its range on the calling thread. */
+static void parallel_range(uint64_t n, int nthreads, range_fn fn, void *ctx) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > MAXTHREADS) nthreads = MAXTHREADS;
+  pthread_t th[MAXTHREADS];
+  int started[MAXTHREADS] = {0};
+  struct range_job jobs[MAXTHREADS];
+  for (int t = 0; t < nthreads; ++t) {
+    jobs[t].fn = fn;
+    jobs[t].ctx = ctx;
+    jobs[t].b = n * (uint64_t)t / (uint64_t)nthreads;
+    jobs[t].e = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+  }
+  for (int t = 1; t < nthreads; ++t) started[t] = pthread_create(&th[t], NULL, range_thread, &jobs[t]) == 0;
+  range_thread(&jobs[0]);
+  for (int t = 1; t < nthreads; ++t) {
+    if (started[t])
+      pthread_join(th[t], NULL);
+    else
+      range_thread(&jobs[t]);
+  }
+}
+
+/* Threads own contiguous result ranges (their writes never share a cache
+ * line); each result element is read from its source position. */
+struct permute_ctx {
+  const double *src;
+  double *dst;
+  int rank;
+  uint64_t nd[MAXR];    /* result dims */
+  uint64_t jump[MAXR];  /* source stride of result axis p */
+};
+
+static void permute_range(void *p, uint64_t b, uint64_t e) {
+  const struct permute_ctx *c = (const struct permute_ctx *)p;
+  uint64_t idx[MAXR] = {0}, off = 0, rest = b;
+  for (int k = c->rank - 1; k >= 0; --k) {
+    idx[k] = rest % c->nd[k];
+    rest /= c->nd[k];
+    off += idx[k] * c->jump[k];
+  }
+  for (uint64_t t = b; t < e; ++t) {
+    c->dst[2 * t] = c->src[2 * off];
+    c->dst[2 * t + 1] = c->src[2 * off + 1];
+    for (int k = c->rank - 1; k >= 0; --k) {
+      if (++idx[k] < c->nd[k]) { off += c->jump[k]; break; }
+      off -= (c->nd[k] - 1) * c->jump[k];
+      idx[k] = 0;
+    }
+  }
+}
+
+int or_permute_mt(const double *src, double *dst, int rank, const uint64_t *dims, const uint16_t *ptup, int nthreads) {
+  if (rank < 0 || rank > MAXR) return OR_EINVAL;
+  int seen[MAXR] = {0};
+  for (int i = 0; i < rank; ++i) {
+    if (ptup[i] >= rank || seen[ptup[i]]) return OR_EINVAL;
+    seen[ptup[i]] = 1;
+  }
+  struct permute_ctx c;
+  uint64_t st[MAXR];
+  strides_of(rank, dims, st);
+  for (int i = 0; i < rank; ++i) {
+    c.nd[ptup[i]] = dims[i];
+    c.jump[ptup[i]] = st[i];
+  }
+  c.src = src;
+  c.dst = dst;
+  c.rank = rank;
+  parallel_range(or_size(rank, dims), nthreads, permute_range, &c);
+  return OR_OK;
+}
+
+struct gate_ctx {
+  double *state;
+  const double *G;
+  int k, nrest;
+  uint64_t K;
+  uint64_t rest_dims[MAXR], rest_st[MAXR];  /* the non-gate axes, in order */
+  uint64_t gdims[MAXR], gst[MAXR];          /* the gate axes, in gate order */
+};
+
+static void gate_range(void *p, uint64_t b, uint64_t e) {
+  const struct gate_ctx *c = (const struct gate_ctx *)p;
+  const uint64_t K = c->K;
+  uint64_t *off = (uint64_t *)malloc(sizeof(uint64_t) * K);
+  double *in = (double *)malloc(sizeof(double) * 2 * K);
+  if (!off || !in) {
+    free(off);
+    free(in);
+    return;
+  }
+  /* Offsets of the gate inputs i (row-major over the gate axes, last fastest). */
+  for (uint64_t i = 0; i < K; ++i) {
+    uint64_t r = i, o = 0;
+    for (int m = c->k - 1; m >= 0; --m) {
+      o += (r % c->gdims[m]) * c->gst[m];
+      r /= c->gdims[m];
+    }
+    off[i] = o;
+  }
+  for (uint64_t g = b; g < e; ++g) {
+    uint64_t r = g, base = 0;
+    for (int q = c->nrest - 1; q >= 0; --q) {
+      base += (r % c->rest_dims[q]) * c->rest_st[q];
+      r /= c->rest_dims[q];
+    }
+    for (uint64_t i = 0; i < K; ++i) {
+      in[2 * i] = c->state[2 * (base + off[i])];
+      in[2 * i + 1] = c->state[2 * (base + off[i]) + 1];
+    }
+    for (uint64_t o = 0; o < K; ++o) {
+      double accr = 0.0, acci = 0.0;
+      for (uint64_t i = 0; i < K; ++i) {
+        double ar = in[2 * i], ai = in[2 * i + 1];
+        double br = c->G[2 * (i * K + o)], bi = c->G[2 * (i * K + o) + 1];
+        double pr = ar * br - ai * bi;
+        double pi = ar * bi + ai * br;
+        accr = accr + pr;
+        acci = acci + pi;
+      }
+      c->state[2 * (base + off[o])] = accr;
+      c->state[2 * (base + off[o]) + 1] = acci;
+    }
+  }
+  free(off);
+  free(in);
+}
+
+int or_gate_apply_mt(double *state, int rank, const uint64_t *dims, int k, const uint16_t *axes, const double *G,
+                     int nthreads) {
+  if (k <= 0 || k > rank || rank > MAXR) return OR_EINVAL;
+  struct gate_ctx c;
+  uint64_t st[MAXR];
+  int is_gate[MAXR] = {0};
+  strides_of(rank, dims, st);
+  c.K = 1;
+  for (int m = 0; m < k; ++m) {
+    if (axes[m] >= rank || is_gate[axes[m]]) return OR_EINVAL;
+    is_gate[axes[m]] = 1;
+    c.gdims[m] = dims[axes[m]];
+    c.gst[m] = st[axes[m]];
+    c.K *= dims[axes[m]];
+  }
+  c.nrest = 0;
+  uint64_t groups = 1;
+  for (int i = 0; i < rank; ++i)
+    if (!is_gate[i]) {
+      c.rest_dims[c.nrest] = dims[i];
+      c.rest_st[c.nrest] = st[i];
+      ++c.nrest;
+      groups *= dims[i];
+    }
+  c.state = state;
+  c.G = G;
+  c.k = k;
+  parallel_range(groups, nthreads, gate_range, &c);
+  return OR_OK;
+}
+
+struct diag_ctx {
+  double *state;
+  const double *d;
+  int k;
+  const uint64_t *dims;
+  const uint16_t *axes;
+  uint64_t st[MAXR], dst[MAXR];
+};
+
+static void diag_range(void *p, uint64_t b, uint64_t e) {
+  const struct diag_ctx *c = (const struct diag_ctx *)p;
+  for (uint64_t x = b; x < e; ++x) {
+    uint64_t di = 0;
+    for (int m = 0; m < c->k; ++m) di += ((x / c->st[c->axes[m]]) % c->dims[c->axes[m]]) * c->dst[m];
+    double ar = c->state[2 * x], ai = c->state[2 * x + 1];
+    double br = c->d[2 * di], bi = c->d[2 * di + 1];
+    c->state[2 * x] = ar * br - ai * bi;
+    c->state[2 * x + 1] = ar * bi + ai * br;
+  }
+}
+
+int or_diag_apply_mt(double *state, int rank, const uint64_t *dims, int k, const uint16_t *axes, const double *d,
+                     int nthreads) {
+  if (k <= 0 || k > rank || rank > MAXR) return OR_EINVAL;
+  struct diag_ctx c;
+  strides_of(rank, dims, c.st);
+  uint64_t s = 1;
+  for (int m = k - 1; m >= 0; --m) {
+    if (axes[m] >= rank) return OR_EINVAL;
+    c.dst[m] = s;
+    s *= dims[axes[m]];
+  }
+  c.state = state;
+  c.d = d;
+  c.k = k;
+  c.dims = dims;
+  c.axes = axes;
+  parallel_range(or_size(rank, dims), nthreads, diag_range, &c);
+  return OR_OK;
+}

@@ -97,6 +97,16 @@ int or_diag_apply(double *state, int rank, const uint64_t *dims, int k, const ui
  * host math library. */
 int or_qft(double *state, int n, const double *phases);
 
+/* ---- CPU timing baseline (not checkers) ------------------------------------ */
+/* The same per-element arithmetic as or_permute, or_gate_apply and
+ * or_diag_apply, split over nthreads POSIX threads; results equal theirs bit
+ * for bit. */
+int or_permute_mt(const double *src, double *dst, int rank, const uint64_t *dims, const uint16_t *ptup, int nthreads);
+int or_gate_apply_mt(double *state, int rank, const uint64_t *dims, int k, const uint16_t *axes, const double *G,
+                     int nthreads);
+int or_diag_apply_mt(double *state, int rank, const uint64_t *dims, int k, const uint16_t *axes, const double *d,
+                     int nthreads);
+
 /* ---- reductions used by property checks ------------------------------------ */
 double or_norm2(const double *x, uint64_t n);
 /* Position-independent multiset hash of the element bit patterns. */

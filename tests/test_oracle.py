@@ -374,3 +374,26 @@ def test_multiset_hash_is_permutation_invariant():
     assert oracle.multiset_hash(x) == oracle.multiset_hash(y)
     y[5] += 1e-9
     assert oracle.multiset_hash(x) != oracle.multiset_hash(y)
+
+
+# ------------------------------------------------------------------ threaded CPU baseline
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_baseline_equals_oracle_bitwise(threads):
+    """The multi-threaded CPU baseline does the oracle's arithmetic element for
+    element: permute, gate (1-3 axes, mixed dims) and diagonal agree bitwise."""
+    rng = np.random.default_rng(threads)
+    for dims in ([2] * 12, [2, 3, 2, 4, 5, 2]):
+        n = int(np.prod(dims))
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        p = list(rng.permutation(len(dims)))
+        assert np.array_equal(oracle.permute_mt(x, dims, p, threads).view(np.uint64),
+                              oracle.permute(x, dims, p).view(np.uint64))
+        for k in (1, 2, 3):
+            axes = [int(a) for a in rng.choice(len(dims), k, replace=False)]
+            kk = int(np.prod([dims[a] for a in axes]))
+            g = rng.standard_normal(kk * kk) + 1j * rng.standard_normal(kk * kk)
+            assert np.array_equal(oracle.gate_apply_mt(x, dims, axes, g, threads).view(np.uint64),
+                                  oracle.gate_apply(x, dims, axes, g).view(np.uint64)), (dims, axes)
+            dv = rng.standard_normal(kk) + 1j * rng.standard_normal(kk)
+            assert np.array_equal(oracle.diag_apply_mt(x, dims, axes, dv, threads).view(np.uint64),
+                                  oracle.diag_apply(x, dims, axes, dv).view(np.uint64)), (dims, axes)
