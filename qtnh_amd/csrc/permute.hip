@@ -408,19 +408,20 @@ void launch_bits(hipStream_t stream, int num_cus, const c128 *src, c128 *dst, co
 }
 
 // Times the candidate tile shapes for one signature on the caller's buffers and
-// returns the fastest; *wrote is set when dst already holds the result.
+// returns the fastest; *wrote is set when dst already holds the result.  Every
+// shape is timed once, then the two fastest again (the better second timing
+// wins), so one noisy timing does not decide.
 qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sigma, const int *inv, int nb,
                       PermTune *best, bool *wrote) {
   *wrote = false;
-  float best_ms = 0.0f;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t err = hipEventCreate(&e0);
   if (err == hipSuccess) err = hipEventCreate(&e1);
-  for (const PermTune &cand : kPermCandidates) {
-    if (err != hipSuccess) break;
+  // Time of kPermTimedRuns launches of shape cand (after one warm-up launch); < 0 if it does not apply.
+  auto time_shape = [&](const PermTune &cand) -> float {
     PermBitsArgs args;
     int u, no;
-    if (!plan_bits(sigma, inv, nb, cand, args, &u, &no)) continue;
+    if (err != hipSuccess || !plan_bits(sigma, inv, nb, cand, args, &u, &no)) return -1.0f;
     launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cand);  // warm-up; writes the result
     err = hipEventRecord(e0, ctx->stream);
     for (int r = 0; r < kPermTimedRuns && err == hipSuccess; ++r) {
@@ -431,12 +432,31 @@ qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sig
     if (err == hipSuccess) err = hipEventSynchronize(e1);
     float ms = 0.0f;
     if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-    if (err != hipSuccess) break;
+    if (err != hipSuccess) return -1.0f;
     *wrote = true;
-    if (best_ms == 0.0f || ms < best_ms) {
-      best_ms = ms;
-      *best = cand;
+    return ms;
+  };
+  constexpr int kCand = (int)(sizeof(kPermCandidates) / sizeof(kPermCandidates[0]));
+  int first = -1, second = -1;
+  float t_first = 0.0f, t_second = 0.0f;
+  for (int i = 0; i < kCand; ++i) {
+    const float ms = time_shape(kPermCandidates[i]);
+    if (ms < 0.0f) continue;
+    if (first < 0 || ms < t_first) {
+      second = first;
+      t_second = t_first;
+      first = i;
+      t_first = ms;
+    } else if (second < 0 || ms < t_second) {
+      second = i;
+      t_second = ms;
     }
+  }
+  if (first >= 0) *best = kPermCandidates[first];
+  if (first >= 0 && second >= 0) {
+    const float a = time_shape(kPermCandidates[first]);
+    const float b = time_shape(kPermCandidates[second]);
+    if (a >= 0.0f && b >= 0.0f && b < a) *best = kPermCandidates[second];
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
