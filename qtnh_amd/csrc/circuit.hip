@@ -462,17 +462,6 @@ __device__ __forceinline__ PhaseDesc load_phase(const __attribute__((address_spa
   return d;
 }
 
-// Workgroup barrier ordering LDS accesses only: a wave waits for its own LDS
-// operations (lgkmcnt), not for global loads, stores or LDS DMA still in
-// flight on the vector-memory counter.  The "memory" clobbers keep the
-// compiler from moving memory accesses across it.  (A workgroup fence would
-// also wait for the DMA, which writes LDS.)
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 template <int KB, int A, int B, int C>
 __device__ __forceinline__ void reg_gate(double2_t (&v)[16], ccoef_t g) {
   constexpr int K = 1 << KB;

@@ -183,29 +183,143 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_bits(const c128 *
 
 // Power-of-two dims with few open B results on the lowest result bits (a
 // small gate contracted into a large state with its outputs placed last): one
-// thread per row of N = 2^NB results that share their A elements.  It loads
-// the K = 2^KB A terms of its row once and forms every result of the row from
-// them, each as 0 + sum over terms in reference order (the order and
-// arithmetic of contract_direct_bits).
+// thread per row of N = 2^NB results that share their A elements.  Each
+// result is 0 + the sum over its K = 2^KB terms in reference order (the order
+// and arithmetic of contract_direct_bits).
+//
+// A block takes 256 consecutive rows per step.  Its A elements (256 rows x K
+// terms) are read in the order of the A bits they vary in, lowest first, so a
+// wave's loads cover contiguous bytes; they pass through LDS to the thread of
+// their row (K <= 8).  The results pass through LDS too, up to 8 of each row
+// at a time, and are written back so that consecutive lanes store
+// consecutive elements (whole rows, or 128-byte halves of 16-result rows).
+// LDS slots are padded by one per 16 so both sides of each exchange are free
+// of bank conflicts.  nrows is a multiple of 256 (at least 12 row bits), so
+// every thread of a block runs the same steps.
+//
+// The barriers between the exchanges order LDS accesses only: the next
+// step's A loads, issued before this step's results are formed, and this
+// step's C stores stay in flight across them.  The first step is peeled so
+// that the loop is entered with the same loads and stores outstanding as on
+// its back edge (the compiler's wait before the staged values are written to
+// LDS then covers the loads only, not the stores issued after them).  A and C
+// are not restrict-qualified so that the compiler keeps the loads ahead of
+// the barriers and stores.
 template <int NB, int KB>
-__global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *__restrict__ A,
-                                                                    const c128 *__restrict__ B, c128 *__restrict__ C,
+struct RowsThread {
+  static constexpr int N = 1 << NB, K = 1 << KB;
+  static constexpr bool kStageIn = KB <= 3;
+  static constexpr int NS = N > 8 ? 8 : N;  // results of a row staged per pass
+  static constexpr int kSlots = (kStageIn && K > NS ? K : NS) * kDirThreads;
+  // Staged reads: element j of this thread is number e = 256 j + tid of the
+  // block's 256 K A elements in the order of the A bits they vary in
+  // (ascending); off[j] is its A offset within the block, slot[j] its LDS slot.
+  uint64_t off[kStageIn ? K : 1];
+  int slot[kStageIn ? K : 1];
+  // Unstaged reads: A offset of term t within a row.
+  uint64_t da[kStageIn ? 1 : K];
+  // Staged A values in flight (real and imaginary parts apart, so they stay
+  // in registers from one step to the next).
+  double vre[kStageIn ? K : 1], vim[kStageIn ? K : 1];
+};
+
+// A offset of row r's block (the low 8 row bits are zero there).
+__device__ __forceinline__ uint64_t rows_base(const uint64_t (*tab_a)[256], int ntab, uint64_t r) {
+  uint64_t base = 0;
+  for (int t = 1; t < ntab; ++t) base |= tab_a[t][(uint32_t)(r >> (8 * t)) & 255u];
+  return base;
+}
+
+template <int NB, int KB>
+__device__ __forceinline__ void rows_load(RowsThread<NB, KB> &T, const c128 *A, uint64_t base) {
+#pragma unroll
+  for (int j = 0; j < RowsThread<NB, KB>::K; ++j) {
+    const c128 x = A[base | T.off[j]];
+    T.vre[j] = x.re;
+    T.vim[j] = x.im;
+  }
+}
+
+// One step: the block's rows r0 .. r0 + 255.
+template <int NB, int KB>
+__device__ __forceinline__ void rows_step(RowsThread<NB, KB> &T, const c128 *A, c128 *C,
+                                          c128 *stage, const uint64_t (*tab_a)[256], const c128 *bco, int ntab,
+                                          uint64_t r0, uint64_t stride, uint64_t nrows, int tid) {
+  using R = RowsThread<NB, KB>;
+  constexpr int N = R::N, K = R::K, NS = R::NS;
+  c128 a[K];
+  if constexpr (R::kStageIn) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      stage[T.slot[j]].re = T.vre[j];
+      stage[T.slot[j]].im = T.vim[j];
+    }
+    lds_barrier();
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      const int sl = t * kDirThreads + tid;
+      a[t] = stage[sl + (sl >> 4)];
+    }
+    // Next step's loads; the last step reloads its own block (in bounds, unused).
+    const uint64_t r1 = r0 + stride < nrows ? r0 + stride : r0;
+    rows_load(T, A, rows_base(tab_a, ntab, r1));
+    lds_barrier();
+  } else {
+    const uint64_t oa = rows_base(tab_a, ntab, r0) | tab_a[0][tid];
+#pragma unroll
+    for (int t = 0; t < K; ++t) a[t] = A[oa | T.da[t]];
+  }
+#pragma unroll
+  for (int h = 0; h < N / NS; ++h) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const int o = h * NS + q;
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int t = 0; t < K; ++t) cmac(ar, ai, a[t], bco[t * N + o]);
+      c128 v;
+      v.re = ar;
+      v.im = ai;
+      if constexpr (N == 1) {
+        C[r0 + tid] = v;
+      } else {
+        const int sl = tid * NS + q;
+        stage[sl + (sl >> 4)] = v;
+      }
+    }
+    if constexpr (N > 1) {
+      lds_barrier();
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        const int e = j * kDirThreads + tid;
+        C[(r0 + e / NS) * N + h * NS + e % NS] = stage[e + (e >> 4)];
+      }
+      lds_barrier();
+    }
+  }
+}
+
+template <int NB, int KB>
+__global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *A, const c128 *__restrict__ B, c128 *C,
                                                                     OutBits out, ClosedBits cl, uint64_t nrows) {
-  constexpr int N = 1 << NB, K = 1 << KB;
+  using R = RowsThread<NB, KB>;
+  constexpr int N = R::N, K = R::K;
   constexpr int kTabs = 7;
   __shared__ uint64_t tab_a[kTabs][256];
   __shared__ c128 bco[K * N];  // bco[t N + o]: B element of term t for result o
+  __shared__ c128 stage[R::kSlots + R::kSlots / 16];
+  const int tid = threadIdx.x;
   const int rbits = out.nbits - NB;
   const int ntab = (rbits + 7) / 8;
   for (int t = 0; t < ntab; ++t) {
     uint64_t va = 0;
     for (int j = 0; j < 8; ++j) {
       const int p = NB + 8 * t + j;
-      if (p < out.nbits && ((threadIdx.x >> j) & 1u)) va |= 1ull << out.abit[p];
+      if (p < out.nbits && ((tid >> j) & 1)) va |= 1ull << out.abit[p];
     }
-    tab_a[t][threadIdx.x] = va;
+    tab_a[t][tid] = va;
   }
-  for (int i = threadIdx.x; i < K * N; i += kDirThreads) {
+  for (int i = tid; i < K * N; i += kDirThreads) {
     const int t = i / N, o = i % N;
     uint64_t xb = 0;
     for (int c = 0; c < KB; ++c)
@@ -214,34 +328,63 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *
       if ((o >> j) & 1) xb |= 1ull << out.bbit[j];
     bco[i] = B[xb];
   }
-  __syncthreads();
-  // A offset of term t: the single-bit masks of t's set bits.
-  uint64_t da[K];
+  // A bit of each term bit c.
+  int tbit[KB];
 #pragma unroll
-  for (int t = 0; t < K; ++t) {
-    uint64_t x = 0;
+  for (int c = 0; c < KB; ++c) tbit[c] = __builtin_ctzll(cl.fa[c] ^ (c ? cl.fa[c - 1] : 0ull));
+  R T;
+  if constexpr (R::kStageIn) {
+    // Varying bits as (A bit << 8) | (row bit j, or 8 + term bit c), sorted by A bit.
+    int key[8 + KB];
 #pragma unroll
-    for (int c = 0; c < KB; ++c)
-      if ((t >> c) & 1) x |= cl.fa[c] ^ (c ? cl.fa[c - 1] : 0ull);
-    da[t] = x;
-  }
-  for (uint64_t r = (uint64_t)blockIdx.x * kDirThreads + threadIdx.x; r < nrows; r += (uint64_t)gridDim.x * kDirThreads) {
-    uint64_t oa = 0;
-    for (int t = 0; t < ntab; ++t) oa |= tab_a[t][(uint32_t)(r >> (8 * t)) & 255u];
-    c128 a[K];
+    for (int j = 0; j < 8; ++j) key[j] = ((int)out.abit[NB + j] << 8) | j;
 #pragma unroll
-    for (int t = 0; t < K; ++t) a[t] = A[oa | da[t]];
+    for (int c = 0; c < KB; ++c) key[8 + c] = (tbit[c] << 8) | (8 + c);
 #pragma unroll
-    for (int o = 0; o < N; ++o) {
-      double ar = 0.0, ai = 0.0;
+    for (int i = 1; i < 8 + KB; ++i)
 #pragma unroll
-      for (int t = 0; t < K; ++t) cmac(ar, ai, a[t], bco[t * N + o]);
-      c128 v;
-      v.re = ar;
-      v.im = ai;
-      C[r * N + o] = v;
+      for (int m = i; m > 0; --m)
+        if (key[m - 1] > key[m]) {
+          const int x = key[m - 1];
+          key[m - 1] = key[m];
+          key[m] = x;
+        }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int e = j * kDirThreads + tid;
+      uint64_t o = 0;
+      int row = 0, t = 0;
+#pragma unroll
+      for (int i = 0; i < 8 + KB; ++i) {
+        if (!((e >> i) & 1)) continue;
+        o |= 1ull << (key[i] >> 8);
+        const int w = key[i] & 255;
+        if (w < 8)
+          row |= 1 << w;
+        else
+          t |= 1 << (w - 8);
+      }
+      T.off[j] = o;
+      const int sl = t * kDirThreads + row;  // term-major
+      T.slot[j] = sl + (sl >> 4);
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      uint64_t x = 0;
+#pragma unroll
+      for (int c = 0; c < KB; ++c)
+        if ((t >> c) & 1) x |= 1ull << tbit[c];
+      T.da[t] = x;
     }
   }
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * kDirThreads;
+  uint64_t r0 = (uint64_t)blockIdx.x * kDirThreads;
+  if (r0 >= nrows) return;
+  if constexpr (R::kStageIn) rows_load(T, A, rows_base(tab_a, ntab, r0));
+  rows_step(T, A, C, stage, tab_a, bco, ntab, r0, stride, nrows, tid);
+  for (r0 += stride; r0 < nrows; r0 += stride) rows_step(T, A, C, stage, tab_a, bco, ntab, r0, stride, nrows, tid);
 }
 
 // Row kernels exist for NB + KB <= 5 (at most 32 coefficients, held in registers).

@@ -58,6 +58,17 @@ __host__ __device__ inline void cmul_parts(double ar, double ai, double br, doub
   *i = ar * bi + ai * br;
 }
 
+// Workgroup barrier ordering LDS accesses only: a wave waits for its own LDS
+// operations (lgkmcnt), not for global loads, stores or LDS DMA still in
+// flight on the vector-memory counter.  The "memory" clobbers keep the
+// compiler from moving memory accesses across it.  (__syncthreads' workgroup
+// fence also waits for those, and for the DMA, which writes LDS.)
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // Row-major size with overflow detection; false on overflow.
 bool checked_size(int rank, const uint64_t *dims, uint64_t *n);
 
