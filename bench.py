@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--circuit-depth", type=int, default=20)
     p.add_argument("--gemm-n", type=int, default=4096)
     p.add_argument("--networks", type=int, default=256)
-    p.add_argument("--network-max-rank", type=int, default=24)
+    p.add_argument("--network-max-rank", type=int, default=26)
     p.add_argument("--serve-queries", type=int, default=48)
     p.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, queries per second")
     return p.parse_args()
@@ -292,7 +292,9 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
             continue
         nets.append(net)
     info = [n.info() for n in nets]
-    for n in nets[:4]:
+    # Every network once, untimed: first-use costs (permute tile tuning for new
+    # large permutations, context buffers) stay out of both timed passes.
+    for n in nets:
         n.run()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -302,6 +304,9 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
     t2 = time.perf_counter()
     batch, waves = ops.run_networks(nets)
     t3 = time.perf_counter()
+    # The prepared batch below holds all networks at once; the eager pass's
+    # workspace (the same size) is released first.
+    ops.trim()
     # Prepared once, replayed as one HIP graph (repeated evaluation of the same structures).
     prepared = ops.NetworkBatch(nets, use_graph=True)
     prepared.run()

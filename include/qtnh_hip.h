@@ -53,6 +53,12 @@ typedef struct qtnh_ctx_s *qtnh_ctx_t;
  * device's default stream).  The caller keeps ownership of the stream. */
 qtnh_status qtnh_ctx_create(int device, void *stream, qtnh_ctx_t *out);
 qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx);
+/* Releases the device and pinned buffers the context keeps between calls
+ * (contraction workspace, descriptor, reduction and staging buffers) after its
+ * stream has drained; later calls allocate them again as needed.  No reference
+ * counterpart: for callers that hand large amounts of HBM to other work between
+ * batches. */
+qtnh_status qtnh_ctx_trim(qtnh_ctx_t ctx);
 const char *qtnh_status_string(int status);
 const char *qtnh_ctx_last_error(qtnh_ctx_t ctx);
 /* Number of visible HIP devices (0 when none); never fails on a host without GPUs. */

@@ -38,6 +38,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "qtnh_ctx_create": [_i, _p, ctypes.POINTER(_p)],
     "qtnh_ctx_destroy": [_p],
+    "qtnh_ctx_trim": [_p],
     "qtnh_status_string": [_i],
     "qtnh_ctx_last_error": [_p],
     "qtnh_device_count": [ctypes.POINTER(_i)],

@@ -48,6 +48,25 @@ qtnh_status grow_buffer(qtnh_ctx_t ctx, void **buf, uint64_t *cap, uint64_t byte
   return QTNH_OK;
 }
 
+qtnh_status grow_pinned(qtnh_ctx_t ctx, uint64_t bytes) {
+  if (bytes <= ctx->pinned_bytes) return QTNH_OK;
+  if (ctx->pinned) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return check_hip(ctx, e, "pinned buffer sync");
+    (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
+    ctx->pinned_bytes = 0;
+  }
+  const uint64_t want = std::max<uint64_t>(bytes, 1ull << 20);
+  hipError_t e = hipHostMalloc(&ctx->pinned, want);
+  if (e != hipSuccess) {
+    ctx->pinned = nullptr;
+    return check_hip(ctx, e, "pinned host buffer alloc");
+  }
+  ctx->pinned_bytes = want;
+  return QTNH_OK;
+}
+
 qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out) {
   qtnh_status st = grow_buffer(ctx, &ctx->ws, &ctx->ws_bytes, bytes);
   *out = st ? nullptr : ctx->ws;
@@ -285,6 +304,20 @@ qtnh_status qtnh_ctx_create(int device, void *stream, qtnh_ctx_t *out) {
   return QTNH_OK;
 }
 
+qtnh_status qtnh_ctx_trim(qtnh_ctx_t ctx) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "ctx_trim sync");
+  if (st) return st;
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->desc) (void)hipFree(ctx->desc);
+  if (ctx->red) (void)hipFree(ctx->red);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  ctx->ws = ctx->desc = ctx->red = ctx->pinned = nullptr;
+  ctx->ws_bytes = ctx->desc_bytes = ctx->red_bytes = ctx->pinned_bytes = 0;
+  return QTNH_OK;
+}
+
 qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx) {
   if (!ctx) return QTNH_EINVAL;
   // Teardown is best effort: errors here cannot be reported to anyone useful.
@@ -295,6 +328,7 @@ qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx) {
   if (ctx->red) (void)hipFree(ctx->red);
   if (ctx->dev_scalar) (void)hipFree(ctx->dev_scalar);
   if (ctx->host_scalar) (void)hipHostFree(ctx->host_scalar);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;
   return QTNH_OK;
 }

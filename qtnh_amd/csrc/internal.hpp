@@ -27,6 +27,9 @@ struct qtnh_ctx_s {
   // Growable device buffer for reduction partials (expectation values).
   void *red = nullptr;
   uint64_t red_bytes = 0;
+  // Growable pinned host buffer staging descriptor uploads (asynchronous copies).
+  void *pinned = nullptr;
+  uint64_t pinned_bytes = 0;
   // Fused-circuit tile shape chosen by the autotuner, per qubit count (0: unset).
   int8_t circ_tb[64] = {0};
   int8_t circ_low[64] = {0};
@@ -48,6 +51,9 @@ qtnh_status workspace(qtnh_ctx_t ctx, uint64_t bytes, void **out);
 // Grows a context-owned device buffer (*buf, *cap) to at least `bytes`; the old
 // buffer is freed only after the stream has drained.
 qtnh_status grow_buffer(qtnh_ctx_t ctx, void **buf, uint64_t *cap, uint64_t bytes);
+// Grows the context's pinned host staging buffer to at least `bytes`; the old
+// buffer is freed only after the stream has drained (queued copies read it).
+qtnh_status grow_pinned(qtnh_ctx_t ctx, uint64_t bytes);
 // Makes the context's device current for the calling thread.
 qtnh_status enter_ctx(qtnh_ctx_t ctx);
 

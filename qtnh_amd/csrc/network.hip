@@ -159,12 +159,17 @@ struct qtnh_net_s {
   uint64_t arena_elems = 0, ws_bytes = 0;
   int max_rank = 0;
   double flops = 0, bytes = 0;
+  // Grouped-launch descriptor of each direct power-of-two step, without its
+  // device pointers and launch position (filled in per batch placement).
+  std::vector<GDesc> gdesc;
 };
 
 namespace qtnh_hip {
 namespace {
 
 uint64_t block_elems(const qtnh_net_s &n) { return round16(n.leaf_elems) + round16(n.arena_elems); }
+
+GDesc gdesc_template(const qtnh_net_s &net, size_t s);
 
 // First-fit arena over element offsets; returns the high-water mark.
 struct Arena {
@@ -344,6 +349,9 @@ qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
   }
   net.arena_elems = hw;
   net.result = net.steps.empty() ? 0 : net.steps.back().c;
+  net.gdesc.assign(net.steps.size(), GDesc{});
+  for (size_t s = 0; s < net.steps.size(); ++s)
+    if (net.steps[s].direct && net.steps[s].binary) net.gdesc[s] = gdesc_template(net, s);
   net.planned = true;
   return QTNH_OK;
 }
@@ -370,16 +378,16 @@ qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t 
                          node_ptr(net, block, st.c), QTNH_PREC_FP64, nullptr, ws, ws_bytes);
 }
 
-GDesc make_gdesc(const qtnh_net_s &net, Where block, size_t s, uint64_t first) {
+GDesc gdesc_template(const qtnh_net_s &net, size_t s) {
   const auto &st = net.steps[s];
   const auto &A = net.nodes[st.a], &B = net.nodes[st.b], &C = net.nodes[st.c];
   const int rA = (int)A.dims.size(), rB = (int)B.dims.size(), rC = (int)C.dims.size();
   GDesc d;
   memset(&d, -1, sizeof(d));
-  d.A = (const double2_t *)node_ptr(net, block, st.a);
-  d.B = (const double2_t *)node_ptr(net, block, st.b);
-  d.C = (double2_t *)node_ptr(net, block, st.c);
-  d.first = first;
+  d.A = nullptr;
+  d.B = nullptr;
+  d.C = nullptr;
+  d.first = 0;
   d.nC = C.size;
   d.ncb = rC;
   const int nw = (int)st.wires.size() / 2;
@@ -415,6 +423,17 @@ GDesc make_gdesc(const qtnh_net_s &net, Where block, size_t s, uint64_t first) {
   return d;
 }
 
+// The step's descriptor for one placement of the network.
+GDesc make_gdesc(const qtnh_net_s &net, Where block, size_t s, uint64_t first) {
+  const auto &st = net.steps[s];
+  GDesc d = net.gdesc[s];
+  d.A = (const double2_t *)node_ptr(net, block, st.a);
+  d.B = (const double2_t *)node_ptr(net, block, st.b);
+  d.C = (double2_t *)node_ptr(net, block, st.c);
+  d.first = first;
+  return d;
+}
+
 uint64_t gdesc_blocks(const GDesc &d) { return (d.nC + kGThreads - 1) / kGThreads; }
 
 }  // namespace
@@ -423,6 +442,9 @@ uint64_t gdesc_blocks(const GDesc &d) { return (d.nC + kGThreads - 1) / kGThread
 namespace qtnh_hip {
 namespace {
 
+// Descriptors per upload-and-launch range of an eager batch pass.
+constexpr size_t kDescChunk = 2048;
+
 // One batch of networks placed in one device block:
 //   [leaves of every net | arenas | packed results | GEMM workspace]
 // plus descriptor memory (grouped-step descriptors, result gather table).
@@ -430,7 +452,9 @@ struct Wave {
   std::vector<const qtnh_net_s *> nets;
   std::vector<Where> blocks;
   std::vector<double> leaves;  // host copy of every leaf, packed as on the device
-  std::vector<GDesc> descs;
+  std::vector<GDesc> descs;    // host copy, when built in full (prepared batches)
+  const GDesc *hdesc = nullptr;  // host copy in use (descs or the context's pinned staging)
+  size_t ndesc = 0;
   std::vector<size_t> starts;  // descriptors of step index s: [starts[s], starts[s + 1])
   std::vector<Gather> gathers;
   c128 *base = nullptr, *res = nullptr;
@@ -450,7 +474,10 @@ struct Wave {
     return (elems + round16(res)) * sizeof(c128) + ws;
   }
 
-  void layout(qtnh_ctx_t, const qtnh_net_t *ns, int count, char *mem) {
+  // Placement of every network in `mem`, the leaf image, the descriptor count
+  // of each step index and the result gather table; with `with_descs` also
+  // every descriptor (else fill_descs builds them step range by step range).
+  void layout(const qtnh_net_t *ns, int count, char *mem, bool with_descs) {
     nets.assign(ns, ns + count);
     uint64_t elems = 0;
     for (auto *n : nets) {
@@ -472,18 +499,15 @@ struct Wave {
       aoff += round16(n->arena_elems);
     }
     starts.assign(steps + 1, 0);
-    for (size_t s = 0; s < steps; ++s) {
-      starts[s] = descs.size();
-      uint64_t first = 0;
-      for (size_t i = 0; i < nets.size(); ++i) {
-        const auto &n = *nets[i];
-        if (s >= n.steps.size() || !n.steps[s].direct || !n.steps[s].binary) continue;
-        GDesc d = make_gdesc(n, blocks[i], s, first);
-        first += gdesc_blocks(d);
-        descs.push_back(d);
-      }
+    for (auto *n : nets)
+      for (size_t s = 0; s < n->steps.size(); ++s) starts[s + 1] += n->steps[s].direct && n->steps[s].binary;
+    for (size_t s = 0; s < steps; ++s) starts[s + 1] += starts[s];
+    ndesc = starts[steps];
+    if (with_descs) {
+      descs.resize(ndesc);
+      fill_descs(0, steps, descs.data());
+      hdesc = descs.data();
     }
-    starts[steps] = descs.size();
     uint64_t dst = 0;
     for (size_t i = 0; i < nets.size(); ++i) {
       const auto &n = *nets[i];
@@ -492,16 +516,32 @@ struct Wave {
     }
   }
 
+  // Descriptors of step indices [s0, s1) into out[starts[s0] .. starts[s1]).
+  void fill_descs(size_t s0, size_t s1, GDesc *out) const {
+    for (size_t s = s0; s < s1; ++s) {
+      size_t k = starts[s];
+      uint64_t first = 0;
+      for (size_t i = 0; i < nets.size(); ++i) {
+        const auto &n = *nets[i];
+        if (s >= n.steps.size() || !n.steps[s].direct || !n.steps[s].binary) continue;
+        out[k] = make_gdesc(n, blocks[i], s, first);
+        first += gdesc_blocks(out[k]);
+        ++k;
+      }
+    }
+  }
+
   uint64_t desc_bytes() const {
-    return ((descs.size() * sizeof(GDesc) + 255) & ~255ull) + gathers.size() * sizeof(Gather) + 256;
+    return ((ndesc * sizeof(GDesc) + 255) & ~255ull) + gathers.size() * sizeof(Gather) + 256;
   }
 
   void place_descriptors(void *mem) {
     ddesc = (GDesc *)mem;
-    dgath = (Gather *)((char *)mem + ((descs.size() * sizeof(GDesc) + 255) & ~255ull));
+    dgath = (Gather *)((char *)mem + ((ndesc * sizeof(GDesc) + 255) & ~255ull));
   }
 
-  // Leaves and descriptors to the device (stream-ordered).
+  // Leaves, the gather table and (when built in full) the descriptors to the
+  // device (stream-ordered).
   qtnh_status upload(qtnh_ctx_t ctx) {
     qtnh_status st = check_hip(
         ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
@@ -520,10 +560,17 @@ struct Wave {
   // Every kernel of the batch, in step order, then the result gather.  Only
   // kernel launches (no copies, fills or allocations): capturable.
   qtnh_status enqueue(qtnh_ctx_t ctx) {
-    for (size_t s = 0; s < steps; ++s) {
+    qtnh_status st = enqueue_steps(ctx, 0, steps);
+    if (st) return st;
+    return enqueue_gather(ctx);
+  }
+
+  // The kernels of step indices [s0, s1).
+  qtnh_status enqueue_steps(qtnh_ctx_t ctx, size_t s0, size_t s1) {
+    for (size_t s = s0; s < s1; ++s) {
       const size_t b = starts[s], e = starts[s + 1];
       if (e > b) {
-        const uint64_t tb = descs[e - 1].first + gdesc_blocks(descs[e - 1]);
+        const uint64_t tb = hdesc[e - 1].first + gdesc_blocks(hdesc[e - 1]);
         const unsigned grid = (unsigned)std::min<uint64_t>(tb, (uint64_t)ctx->num_cus * 16);
         hipLaunchKernelGGL(grouped_direct_kernel, dim3(grid), dim3(kGThreads), 0, ctx->stream, ddesc + b, (int)(e - b),
                            tb);
@@ -537,6 +584,10 @@ struct Wave {
         if (st) return st;
       }
     }
+    return QTNH_OK;
+  }
+
+  qtnh_status enqueue_gather(qtnh_ctx_t ctx) {
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)gathers.size()), dim3(64), 0, ctx->stream, dgath,
                        (int)gathers.size(), (double2_t *)res);
     return check_hip(ctx, hipGetLastError(), "gather launch");
@@ -757,12 +808,30 @@ qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets
     st = workspace(ctx, bytes, &wp);
     if (st) return st;
     Wave w;
-    w.layout(ctx, nets + w0, w1 - w0, (char *)wp);
+    w.layout(nets + w0, w1 - w0, (char *)wp, false);
     st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, w.desc_bytes());
+    if (!st) st = grow_pinned(ctx, std::max<size_t>(w.ndesc, 1) * sizeof(GDesc));
     if (st) return st;
     w.place_descriptors(ctx->desc);
     st = w.upload(ctx);
-    if (!st) st = w.enqueue(ctx);
+    // Descriptors are built, copied (from pinned memory, asynchronously) and
+    // launched a range of step indices at a time, so the GPU starts on the
+    // first steps while the host builds the rest.  Each range's staging is
+    // written once and read by its copy before the wave's closing sync.
+    GDesc *hp = (GDesc *)ctx->pinned;
+    w.hdesc = hp;
+    for (size_t s0 = 0, s1; !st && s0 < w.steps; s0 = s1) {
+      s1 = s0 + 1;
+      while (s1 < w.steps && w.starts[s1] - w.starts[s0] < kDescChunk) ++s1;
+      w.fill_descs(s0, s1, hp);
+      const size_t b = w.starts[s0], e = w.starts[s1];
+      if (e > b)
+        st = check_hip(ctx, hipMemcpyAsync(w.ddesc + b, hp + b, (e - b) * sizeof(GDesc), hipMemcpyHostToDevice,
+                                           ctx->stream),
+                       "batch descriptor upload");
+      if (!st) st = w.enqueue_steps(ctx, s0, s1);
+    }
+    if (!st) st = w.enqueue_gather(ctx);
     if (!st) st = w.fetch(ctx, outs + w0);
     if (st) return st;
     w0 = w1;
@@ -791,7 +860,7 @@ qtnh_status qtnh_net_batch_create(qtnh_ctx_t ctx, int count, const qtnh_net_t *n
   // Buffers of its own: a captured graph holds their addresses.
   st = check_hip(ctx, hipMalloc(&b->mem, Wave::device_bytes(nets, count)), "net_batch_create alloc");
   if (st) return fail(st);
-  b->wave.layout(ctx, nets, count, (char *)b->mem);
+  b->wave.layout(nets, count, (char *)b->mem, true);
   st = check_hip(ctx, hipMalloc(&b->desc, b->wave.desc_bytes()), "net_batch_create alloc");
   if (st) return fail(st);
   b->wave.place_descriptors(b->desc);

@@ -36,6 +36,13 @@ def context(device=None, stream=None):
     return ctx
 
 
+def trim(device=None, stream=None):
+    """Release the context's cached device and pinned buffers (they are
+    allocated again when needed)."""
+    ctx = context(device, stream)
+    check(lib().qtnh_ctx_trim(ctx), ctx)
+
+
 def _u64(v):
     a = np.ascontiguousarray(np.asarray(list(v), dtype=np.uint64))
     if a.size == 0:
