@@ -113,3 +113,24 @@ def test_batch_waves_under_a_small_budget():
     assert waves > 1
     for s, b in zip(solo, batch):
         assert same_bits(s, b)
+
+
+def test_batch_over_many_step_ranges_and_after_trim():
+    """Enough networks that the eager pass builds, uploads and launches its
+    descriptors over several step ranges (2048 descriptors each); results equal
+    one-at-a-time runs bit for bit, also after the context's cached buffers
+    are released (qtnh_ctx_trim) and allocated again."""
+    nets = []
+    for i in range(40):
+        circ = ops.Circuit(16).grid(4, 4, 8 + i % 5, 900 + i)
+        bits = np.random.default_rng(100 + i).integers(0, 2, 16)
+        nets.append(ops.Network.amplitude(circ, bits).plan(i))
+    assert sum(n.info()["steps"] for n in nets) > 3 * 2048
+    solo = [n.run() for n in nets]
+    batch, waves = ops.run_networks(nets)
+    assert waves == 1
+    ops.trim()
+    again, _ = ops.run_networks(nets)
+    for s, b, a in zip(solo, batch, again):
+        assert same_bits(s, b)
+        assert same_bits(s, a)
