@@ -316,6 +316,20 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
         replay = prepared.run()
     t5 = time.perf_counter()
     prepared.close()
+    # The same batch with the GEMM-route steps in split precision (small steps
+    # stay exact FP64); one untimed pass first (permute tuning for the split
+    # route's operand layouts), error per network against the FP64 results.
+    for n in nets:
+        n.set_precision(_lib.PREC_SPLIT)
+    ops.run_networks(nets)
+    torch.cuda.synchronize()
+    t6 = time.perf_counter()
+    split, _ = ops.run_networks(nets)
+    t7 = time.perf_counter()
+    for n in nets:
+        n.set_precision(_lib.PREC_FP64)
+    ops.trim()
+    rel = [float(np.abs(s - b).max() / max(np.abs(b).max(), 1e-300)) for s, b in zip(split, batch)]
     flops = sum(i["flops"] for i in info)
     return {"networks": count, "redrawn_over_rank_cap": redrawn, "rank_cap": max_rank,
             "max_rank": max(i["max_rank"] for i in info), "steps": sum(i["steps"] for i in info),
@@ -325,6 +339,10 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
             "batched_GFLOP_per_s": round(flops / (t3 - t2) / 1e9, 1),
             "graph_replay_ms": round((t5 - t4) / reps * 1e3, 1),
             "graph_replay_speedup": round((t1 - t0) / ((t5 - t4) / reps), 2),
+            "split": {"batched_ms": round((t7 - t6) * 1e3, 1),
+                      "speedup_vs_one_at_a_time_fp64": round((t1 - t0) / (t7 - t6), 2),
+                      "speedup_vs_batched_fp64": round((t3 - t2) / (t7 - t6), 2),
+                      "max_rel_err": max(rel), "median_rel_err": float(np.median(rel))},
             "bitwise_equal": all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, batch))
             and all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, replay))}
 

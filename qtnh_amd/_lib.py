@@ -83,6 +83,7 @@ SIGNATURES = {
     "qtnh_net_add_tensor": [_p, _i, _u64p, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(_i)],
     "qtnh_net_from_circuit": [_p, _p, ctypes.POINTER(ctypes.c_uint8)],
     "qtnh_net_plan": [_p, _u64, _i],
+    "qtnh_net_set_precision": [_p, _i],
     "qtnh_net_info": [_p, _u64p, _u64p, ctypes.POINTER(_i), _dp, _dp, _u64p],
     "qtnh_net_result_dims": [_p, ctypes.POINTER(_i), _u64p],
     "qtnh_net_step": [_p, _u64, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],

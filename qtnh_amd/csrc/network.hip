@@ -158,6 +158,7 @@ struct qtnh_net_s {
   int result = -1;
   uint64_t arena_elems = 0, ws_bytes = 0;
   int max_rank = 0;
+  int precision = QTNH_PREC_FP64;  // of the GEMM-route steps
   double flops = 0, bytes = 0;
   // Grouped-launch descriptor of each direct power-of-two step, without its
   // device pointers and launch position (filled in per batch placement).
@@ -170,6 +171,16 @@ namespace {
 uint64_t block_elems(const qtnh_net_s &n) { return round16(n.leaf_elems) + round16(n.arena_elems); }
 
 GDesc gdesc_template(const qtnh_net_s &net, size_t s);
+
+// Route launch_contract takes for a step (default placement) at a precision.
+ContractRoute step_route(const qtnh_net_s &net, const qtnh_net_s::Step &st, int precision) {
+  const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
+  std::vector<uint16_t> r1(std::max<size_t>(A.dims.size(), 1)), r2(std::max<size_t>(B.dims.size(), 1));
+  const int nw = (int)st.wires.size() / 2;
+  default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
+  return contract_route((int)A.dims.size(), A.dims.data(), (int)B.dims.size(), B.dims.data(), nw, st.wires.data(),
+                        r1.data(), r2.data(), precision);
+}
 
 // First-fit arena over element offsets; returns the high-water mark.
 struct Arena {
@@ -311,11 +322,7 @@ qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
     for (uint64_t d : B.dims) st.binary = st.binary && d == 2;
     st.binary = st.binary && A.dims.size() <= kGMaxBits && B.dims.size() <= kGMaxBits && r.dims.size() <= kGMaxBits;
     // Route and workspace, as launch_contract will choose them.
-    std::vector<uint16_t> r1(A.dims.size()), r2(B.dims.size());
-    default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, (int)st.wires.size() / 2, st.wires.data(), r1.data(),
-                  r2.data());
-    ContractRoute R = contract_route((int)A.dims.size(), A.dims.data(), (int)B.dims.size(), B.dims.data(),
-                                     (int)st.wires.size() / 2, st.wires.data(), r1.data(), r2.data(), QTNH_PREC_FP64);
+    const ContractRoute R = step_route(net, st, net.precision);
     st.direct = R.direct;
     st.ws = R.ws_bytes;
     net.ws_bytes = std::max(net.ws_bytes, st.ws);
@@ -375,7 +382,7 @@ qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t 
   default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
   return launch_contract(ctx, node_ptr(net, block, st.a), (int)A.dims.size(), A.dims.data(), node_ptr(net, block, st.b),
                          (int)B.dims.size(), B.dims.data(), nw, st.wires.data(), r1.data(), r2.data(),
-                         node_ptr(net, block, st.c), QTNH_PREC_FP64, nullptr, ws, ws_bytes);
+                         node_ptr(net, block, st.c), net.precision, nullptr, ws, ws_bytes);
 }
 
 GDesc gdesc_template(const qtnh_net_s &net, size_t s) {
@@ -720,6 +727,19 @@ qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_
 qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank) {
   if (!net || max_rank < 0) return QTNH_EINVAL;
   return plan_greedy(*net, seed, max_rank);
+}
+
+qtnh_status qtnh_net_set_precision(qtnh_net_t net, int precision) {
+  if (!net || (precision != QTNH_PREC_FP64 && precision != QTNH_PREC_SPLIT)) return QTNH_EINVAL;
+  net->precision = precision;
+  if (!net->planned) return QTNH_OK;
+  // The order and the routes stay; the GEMM steps' workspace follows the precision.
+  net->ws_bytes = 0;
+  for (auto &st : net->steps) {
+    if (!st.direct) st.ws = step_route(*net, st, precision).ws_bytes;
+    net->ws_bytes = std::max(net->ws_bytes, st.ws);
+  }
+  return QTNH_OK;
 }
 
 qtnh_status qtnh_net_info(qtnh_net_t net, uint64_t *n_tensors, uint64_t *n_steps, int *max_rank, double *flops,

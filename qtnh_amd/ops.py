@@ -441,6 +441,11 @@ class Network:
         check(lib().qtnh_net_plan(self._h, seed, max_rank))
         return self
 
+    def set_precision(self, precision):
+        """_lib.PREC_FP64 (default) or _lib.PREC_SPLIT for the GEMM-route steps."""
+        check(lib().qtnh_net_set_precision(self._h, precision))
+        return self
+
     def info(self):
         nt = ctypes.c_uint64(0)
         ns = ctypes.c_uint64(0)
