@@ -187,15 +187,16 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_bits(const c128 *
 // result is 0 + the sum over its K = 2^KB terms in reference order (the order
 // and arithmetic of contract_direct_bits).
 //
-// A block takes 256 consecutive rows per step.  Its A elements (256 rows x K
-// terms) are read in the order of the A bits they vary in, lowest first, so a
-// wave's loads cover contiguous bytes; they pass through LDS to the thread of
-// their row (K <= 8).  The results pass through LDS too, up to 8 of each row
-// at a time, and are written back so that consecutive lanes store
-// consecutive elements (whole rows, or 128-byte halves of 16-result rows).
-// LDS slots are padded by one per 16 so both sides of each exchange are free
-// of bank conflicts.  nrows is a multiple of 256 (at least 12 row bits), so
-// every thread of a block runs the same steps.
+// A block takes 256 R consecutive rows per step (R = 2^RB rows per thread).
+// Its A elements (256 R rows x K terms) are read in the order of the A bits
+// they vary in, lowest first, so a wave's loads cover contiguous bytes; they
+// pass through LDS to the thread of their row (K <= 8).  The results pass
+// through LDS too, up to 8 of each row at a time, and are written back so
+// that consecutive lanes store consecutive elements (whole rows, or 128-byte
+// halves of 16-result rows).  LDS slots are padded by one per 16 so both
+// sides of each exchange are free of bank conflicts.  nrows is a multiple of
+// 256 R (at least 12 row bits, RB <= 1), so every thread of a block runs the
+// same steps.
 //
 // The barriers between the exchanges order LDS accesses only: the next
 // step's A loads, issued before this step's results are formed, and this
@@ -206,24 +207,35 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_bits(const c128 *
 // are not restrict-qualified so that the compiler keeps the loads ahead of
 // the barriers and stores.
 template <int NB, int KB>
-struct RowsThread {
+struct RowsShape {
   static constexpr int N = 1 << NB, K = 1 << KB;
   static constexpr bool kStageIn = KB <= 3;
   static constexpr int NS = N > 8 ? 8 : N;  // results of a row staged per pass
-  static constexpr int kSlots = (kStageIn && K > NS ? K : NS) * kDirThreads;
-  // Staged reads: element j of this thread is number e = 256 j + tid of the
-  // block's 256 K A elements in the order of the A bits they vary in
-  // (ascending); off[j] is its A offset within the block, slot[j] its LDS slot.
-  uint64_t off[kStageIn ? K : 1];
-  int slot[kStageIn ? K : 1];
-  // Unstaged reads: A offset of term t within a row.
-  uint64_t da[kStageIn ? 1 : K];
-  // Staged A values in flight (real and imaginary parts apart, so they stay
-  // in registers from one step to the next).
-  double vre[kStageIn ? K : 1], vim[kStageIn ? K : 1];
+  // Two rows per thread where both exchanges fit 2048 LDS slots (32 KiB):
+  // twice the loads in flight per thread.
+  static constexpr int RB = kStageIn && (K > NS ? K : NS) <= 4 ? 1 : 0;
+  static constexpr int R = 1 << RB;
+  static constexpr int kRows = kDirThreads * R;  // rows per block step
+  static constexpr int kSlots = (kStageIn && K > NS ? K : NS) * kRows;
+  static constexpr int kStaged = kStageIn ? K * R : 1;  // staged A elements per thread
 };
 
-// A offset of row r's block (the low 8 row bits are zero there).
+template <int NB, int KB>
+struct RowsThread : RowsShape<NB, KB> {
+  using S = RowsShape<NB, KB>;
+  // Staged reads: element j of this thread is number e = 256 j + tid of the
+  // block's 256 R K A elements in the order of the A bits they vary in
+  // (ascending); off[j] is its A offset within the block, slot[j] its LDS slot.
+  uint64_t off[S::kStaged];
+  int slot[S::kStaged];
+  // Unstaged reads: A offset of term t within a row.
+  uint64_t da[S::kStageIn ? 1 : S::K];
+  // Staged A values in flight (real and imaginary parts apart, so they stay
+  // in registers from one step to the next).
+  double vre[S::kStaged], vim[S::kStaged];
+};
+
+// A offset of the block step starting at row r (its low 8 + RB row bits are zero).
 __device__ __forceinline__ uint64_t rows_base(const uint64_t (*tab_a)[256], int ntab, uint64_t r) {
   uint64_t base = 0;
   for (int t = 1; t < ntab; ++t) base |= tab_a[t][(uint32_t)(r >> (8 * t)) & 255u];
@@ -233,33 +245,35 @@ __device__ __forceinline__ uint64_t rows_base(const uint64_t (*tab_a)[256], int 
 template <int NB, int KB>
 __device__ __forceinline__ void rows_load(RowsThread<NB, KB> &T, const c128 *A, uint64_t base) {
 #pragma unroll
-  for (int j = 0; j < RowsThread<NB, KB>::K; ++j) {
+  for (int j = 0; j < RowsThread<NB, KB>::kStaged; ++j) {
     const c128 x = A[base | T.off[j]];
     T.vre[j] = x.re;
     T.vim[j] = x.im;
   }
 }
 
-// One step: the block's rows r0 .. r0 + 255.
+// One step: the block's rows r0 .. r0 + 256 R - 1 (this thread's: r0 + tid + 256 q).
 template <int NB, int KB>
 __device__ __forceinline__ void rows_step(RowsThread<NB, KB> &T, const c128 *A, c128 *C,
                                           c128 *stage, const uint64_t (*tab_a)[256], const c128 *bco, int ntab,
                                           uint64_t r0, uint64_t stride, uint64_t nrows, int tid) {
-  using R = RowsThread<NB, KB>;
-  constexpr int N = R::N, K = R::K, NS = R::NS;
-  c128 a[K];
-  if constexpr (R::kStageIn) {
+  using S = RowsShape<NB, KB>;
+  constexpr int N = S::N, K = S::K, NS = S::NS, R = S::R, kRows = S::kRows;
+  c128 a[R][K];
+  if constexpr (S::kStageIn) {
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
+    for (int j = 0; j < S::kStaged; ++j) {
       stage[T.slot[j]].re = T.vre[j];
       stage[T.slot[j]].im = T.vim[j];
     }
     lds_barrier();
 #pragma unroll
-    for (int t = 0; t < K; ++t) {
-      const int sl = t * kDirThreads + tid;
-      a[t] = stage[sl + (sl >> 4)];
-    }
+    for (int q = 0; q < R; ++q)
+#pragma unroll
+      for (int t = 0; t < K; ++t) {
+        const int sl = t * kRows + q * kDirThreads + tid;
+        a[q][t] = stage[sl + (sl >> 4)];
+      }
     // Next step's loads; the last step reloads its own block (in bounds, unused).
     const uint64_t r1 = r0 + stride < nrows ? r0 + stride : r0;
     rows_load(T, A, rows_base(tab_a, ntab, r1));
@@ -267,30 +281,32 @@ __device__ __forceinline__ void rows_step(RowsThread<NB, KB> &T, const c128 *A, 
   } else {
     const uint64_t oa = rows_base(tab_a, ntab, r0) | tab_a[0][tid];
 #pragma unroll
-    for (int t = 0; t < K; ++t) a[t] = A[oa | T.da[t]];
+    for (int t = 0; t < K; ++t) a[0][t] = A[oa | T.da[t]];
   }
 #pragma unroll
   for (int h = 0; h < N / NS; ++h) {
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      const int o = h * NS + q;
-      double ar = 0.0, ai = 0.0;
+    for (int q = 0; q < R; ++q)
 #pragma unroll
-      for (int t = 0; t < K; ++t) cmac(ar, ai, a[t], bco[t * N + o]);
-      c128 v;
-      v.re = ar;
-      v.im = ai;
-      if constexpr (N == 1) {
-        C[r0 + tid] = v;
-      } else {
-        const int sl = tid * NS + q;
-        stage[sl + (sl >> 4)] = v;
+      for (int p = 0; p < NS; ++p) {
+        const int o = h * NS + p;
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int t = 0; t < K; ++t) cmac(ar, ai, a[q][t], bco[t * N + o]);
+        c128 v;
+        v.re = ar;
+        v.im = ai;
+        if constexpr (N == 1) {
+          C[r0 + q * kDirThreads + tid] = v;
+        } else {
+          const int sl = (q * kDirThreads + tid) * NS + p;
+          stage[sl + (sl >> 4)] = v;
+        }
       }
-    }
     if constexpr (N > 1) {
       lds_barrier();
 #pragma unroll
-      for (int j = 0; j < NS; ++j) {
+      for (int j = 0; j < NS * R; ++j) {
         const int e = j * kDirThreads + tid;
         C[(r0 + e / NS) * N + h * NS + e % NS] = stage[e + (e >> 4)];
       }
@@ -302,12 +318,12 @@ __device__ __forceinline__ void rows_step(RowsThread<NB, KB> &T, const c128 *A, 
 template <int NB, int KB>
 __global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *A, const c128 *__restrict__ B, c128 *C,
                                                                     OutBits out, ClosedBits cl, uint64_t nrows) {
-  using R = RowsThread<NB, KB>;
-  constexpr int N = R::N, K = R::K;
+  using S = RowsShape<NB, KB>;
+  constexpr int N = S::N, K = S::K, RB = S::RB;
   constexpr int kTabs = 7;
   __shared__ uint64_t tab_a[kTabs][256];
   __shared__ c128 bco[K * N];  // bco[t N + o]: B element of term t for result o
-  __shared__ c128 stage[R::kSlots + R::kSlots / 16];
+  __shared__ c128 stage[S::kSlots + S::kSlots / 16];
   const int tid = threadIdx.x;
   const int rbits = out.nbits - NB;
   const int ntab = (rbits + 7) / 8;
@@ -332,16 +348,18 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *
   int tbit[KB];
 #pragma unroll
   for (int c = 0; c < KB; ++c) tbit[c] = __builtin_ctzll(cl.fa[c] ^ (c ? cl.fa[c - 1] : 0ull));
-  R T;
-  if constexpr (R::kStageIn) {
-    // Varying bits as (A bit << 8) | (row bit j, or 8 + term bit c), sorted by A bit.
-    int key[8 + KB];
+  RowsThread<NB, KB> T;
+  if constexpr (S::kStageIn) {
+    // Varying bits as (A bit << 8) | w, w = row bit j (< 16) or 16 + term bit c,
+    // sorted by A bit.
+    constexpr int kVary = 8 + RB + KB;
+    int key[kVary];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) key[j] = ((int)out.abit[NB + j] << 8) | j;
+    for (int j = 0; j < 8 + RB; ++j) key[j] = ((int)out.abit[NB + j] << 8) | j;
 #pragma unroll
-    for (int c = 0; c < KB; ++c) key[8 + c] = (tbit[c] << 8) | (8 + c);
+    for (int c = 0; c < KB; ++c) key[8 + RB + c] = (tbit[c] << 8) | (16 + c);
 #pragma unroll
-    for (int i = 1; i < 8 + KB; ++i)
+    for (int i = 1; i < kVary; ++i)
 #pragma unroll
       for (int m = i; m > 0; --m)
         if (key[m - 1] > key[m]) {
@@ -350,22 +368,22 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *
           key[m] = x;
         }
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
+    for (int j = 0; j < S::kStaged; ++j) {
       const int e = j * kDirThreads + tid;
       uint64_t o = 0;
       int row = 0, t = 0;
 #pragma unroll
-      for (int i = 0; i < 8 + KB; ++i) {
+      for (int i = 0; i < kVary; ++i) {
         if (!((e >> i) & 1)) continue;
         o |= 1ull << (key[i] >> 8);
         const int w = key[i] & 255;
-        if (w < 8)
+        if (w < 16)
           row |= 1 << w;
         else
-          t |= 1 << (w - 8);
+          t |= 1 << (w - 16);
       }
       T.off[j] = o;
-      const int sl = t * kDirThreads + row;  // term-major
+      const int sl = t * S::kRows + row;  // term-major
       T.slot[j] = sl + (sl >> 4);
     }
   } else {
@@ -379,10 +397,10 @@ __global__ void __launch_bounds__(kDirThreads) contract_direct_rows(const c128 *
     }
   }
   __syncthreads();
-  const uint64_t stride = (uint64_t)gridDim.x * kDirThreads;
-  uint64_t r0 = (uint64_t)blockIdx.x * kDirThreads;
+  const uint64_t stride = (uint64_t)gridDim.x * S::kRows;
+  uint64_t r0 = (uint64_t)blockIdx.x * S::kRows;
   if (r0 >= nrows) return;
-  if constexpr (R::kStageIn) rows_load(T, A, rows_base(tab_a, ntab, r0));
+  if constexpr (S::kStageIn) rows_load(T, A, rows_base(tab_a, ntab, r0));
   rows_step(T, A, C, stage, tab_a, bco, ntab, r0, stride, nrows, tid);
   for (r0 += stride; r0 < nrows; r0 += stride) rows_step(T, A, C, stage, tab_a, bco, ntab, r0, stride, nrows, tid);
 }
@@ -394,7 +412,9 @@ void launch_rows_kb(int kb, dim3 g, hipStream_t st, const c128 *A, const c128 *B
                     const ClosedBits &cb, uint64_t nrows) {
   if constexpr (KB >= 1 && NB + KB <= kRowsMaxBits) {
     if (kb == KB) {
-      hipLaunchKernelGGL((contract_direct_rows<NB, KB>), g, dim3(kDirThreads), 0, st, A, B, C, ob, cb, nrows);
+      const uint64_t steps = nrows / RowsShape<NB, KB>::kRows;
+      const dim3 gk((unsigned)std::min<uint64_t>(std::max<uint64_t>(steps, 1), g.x));
+      hipLaunchKernelGGL((contract_direct_rows<NB, KB>), gk, dim3(kDirThreads), 0, st, A, B, C, ob, cb, nrows);
       return;
     }
     launch_rows_kb<NB, KB - 1>(kb, g, st, A, B, C, ob, cb, nrows);
@@ -523,7 +543,7 @@ qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const 
     const uint64_t cap = (uint64_t)std::max(1, ctx->num_cus) * 16;
     if (b_low_only && cb.kbits >= 1 && nb_low + cb.kbits <= kRowsMaxBits && LC.nbits - nb_low >= 12) {
       const uint64_t nrows = P.nC >> nb_low;
-      const dim3 g((unsigned)std::min<uint64_t>((nrows + kDirThreads - 1) / kDirThreads, cap));
+      const dim3 g((unsigned)cap);  // at most this many blocks, one per block step
       switch (nb_low) {
         case 1: launch_rows_kb<1, 4>(cb.kbits, g, ctx->stream, A, B, C, ob, cb, nrows); break;
         case 2: launch_rows_kb<2, 3>(cb.kbits, g, ctx->stream, A, B, C, ob, cb, nrows); break;
