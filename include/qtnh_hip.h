@@ -205,8 +205,9 @@ qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank);
 /* Arithmetic of the network's large (GEMM-route) steps: QTNH_PREC_FP64 (default)
  * or QTNH_PREC_SPLIT (the split-precision GEMM of qtnh_contract_c128); the small
  * steps always run exactly in FP64 in the reference order.  Before or after
- * planning; also governs the network inside batches created afterwards.  No
- * reference counterpart (the reference contracts in complex<double> only). */
+ * planning.  A prepared batch (qtnh_net_batch_create) keeps the precisions its
+ * networks had when it was created.  No reference counterpart (the reference
+ * contracts in complex<double> only). */
 qtnh_status qtnh_net_set_precision(qtnh_net_t net, int precision);
 /* Plan summary: leaves, steps, largest rank, sum of 8 M N K, sum of
  * 16 (|A| + |B| + |C|) bytes, device bytes of one run (any may be NULL). */

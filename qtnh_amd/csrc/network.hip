@@ -374,7 +374,8 @@ c128 *node_ptr(const qtnh_net_s &net, Where block, int x) {
   return block.arena + net.nodes[x].offset;
 }
 
-qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t s, void *ws, uint64_t ws_bytes) {
+qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t s, void *ws, uint64_t ws_bytes,
+                     int precision) {
   const auto &st = net.steps[s];
   const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
   std::vector<uint16_t> r1(std::max<size_t>(A.dims.size(), 1)), r2(std::max<size_t>(B.dims.size(), 1));
@@ -382,7 +383,7 @@ qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t 
   default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
   return launch_contract(ctx, node_ptr(net, block, st.a), (int)A.dims.size(), A.dims.data(), node_ptr(net, block, st.b),
                          (int)B.dims.size(), B.dims.data(), nw, st.wires.data(), r1.data(), r2.data(),
-                         node_ptr(net, block, st.c), net.precision, nullptr, ws, ws_bytes);
+                         node_ptr(net, block, st.c), precision, nullptr, ws, ws_bytes);
 }
 
 GDesc gdesc_template(const qtnh_net_s &net, size_t s) {
@@ -457,6 +458,7 @@ constexpr size_t kDescChunk = 2048;
 // plus descriptor memory (grouped-step descriptors, result gather table).
 struct Wave {
   std::vector<const qtnh_net_s *> nets;
+  std::vector<int> precs;  // each network's precision when the batch was laid out
   std::vector<Where> blocks;
   std::vector<double> leaves;  // host copy of every leaf, packed as on the device
   std::vector<GDesc> descs;    // host copy, when built in full (prepared batches)
@@ -486,6 +488,7 @@ struct Wave {
   // every descriptor (else fill_descs builds them step range by step range).
   void layout(const qtnh_net_t *ns, int count, char *mem, bool with_descs) {
     nets.assign(ns, ns + count);
+    for (auto *n : nets) precs.push_back(n->precision);
     uint64_t elems = 0;
     for (auto *n : nets) {
       elems += block_elems(*n);
@@ -587,7 +590,7 @@ struct Wave {
       for (size_t i = 0; i < nets.size(); ++i) {
         const auto &n = *nets[i];
         if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary)) continue;
-        qtnh_status st = run_step(ctx, n, blocks[i], s, gws, ws_bytes);
+        qtnh_status st = run_step(ctx, n, blocks[i], s, gws, ws_bytes, precs[i]);
         if (st) return st;
       }
     }
@@ -793,7 +796,7 @@ qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t o
                  "net_run leaf upload");
   if (st) return st;
   for (size_t s = 0; s < net->steps.size(); ++s) {
-    st = run_step(ctx, *net, block, s, ws, net->ws_bytes);
+    st = run_step(ctx, *net, block, s, ws, net->ws_bytes, net->precision);
     if (st) return st;
   }
   st = check_hip(ctx, hipMemcpyAsync(out, node_ptr(*net, block, net->result), rsize * sizeof(c128),

@@ -165,3 +165,22 @@ def test_split_precision_networks():
     back = [n.run() for n in nets]
     for f, b in zip(fp64, back):
         assert same_bits(f, b)
+
+
+def test_prepared_batch_keeps_its_precision():
+    """A batch prepared in FP64 (eager and graph) still gives the FP64 results
+    after its networks are switched to split precision."""
+    from qtnh_amd import _lib
+    circ = ops.Circuit(25).grid(5, 5, 10, 507)
+    net = ops.Network.amplitude(circ, np.random.default_rng(7).integers(0, 2, 25)).plan(7)
+    want = net.run()
+    eager = ops.NetworkBatch([net], use_graph=False)
+    graph = ops.NetworkBatch([net], use_graph=True)
+    net.set_precision(_lib.PREC_SPLIT)
+    try:
+        assert same_bits(eager.run()[0], want)
+        assert same_bits(graph.run()[0], want)
+    finally:
+        net.set_precision(_lib.PREC_FP64)
+        eager.close()
+        graph.close()
