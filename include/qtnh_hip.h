@@ -203,8 +203,10 @@ qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_
  * intermediate.  QTNH_EUNSUPPORTED when an intermediate would exceed max_rank. */
 qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank);
 /* Arithmetic of the network's large (GEMM-route) steps: QTNH_PREC_FP64 (default)
- * or QTNH_PREC_SPLIT (the split-precision GEMM of qtnh_contract_c128); the small
- * steps always run exactly in FP64 in the reference order.  Before or after
+ * or QTNH_PREC_SPLIT (the split-precision GEMM of qtnh_contract_c128, for steps
+ * whose result fills the GPU with at least 256 tiles of 128 x 128 over the
+ * M x 2N real embedding; narrower steps stay FP64); the small steps always run
+ * exactly in FP64 in the reference order.  Before or after
  * planning.  A prepared batch (qtnh_net_batch_create) keeps the precisions its
  * networks had when it was created.  No reference counterpart (the reference
  * contracts in complex<double> only). */

@@ -147,6 +147,7 @@ struct qtnh_net_s {
     bool direct = true;
     bool binary = true;
     uint64_t ws = 0;
+    uint64_t M = 0, N = 0;  // GEMM shape of the route (open A x open B)
     double flops = 0, bytes = 0;
   };
   size_t n_leaves = 0;
@@ -171,6 +172,16 @@ namespace {
 uint64_t block_elems(const qtnh_net_s &n) { return round16(n.leaf_elems) + round16(n.arena_elems); }
 
 GDesc gdesc_template(const qtnh_net_s &net, size_t s);
+
+// A network in split precision uses the split GEMM only for steps whose
+// 128 x 128 output tiles over M x 2N (the embedded real product) cover the
+// 256 CUs; narrower GEMMs (few tiles, long sums) are faster in FP64, where
+// depth chunks fill the chip, and exact.
+constexpr uint64_t kSplitMinTiles = 256;
+int step_precision(const qtnh_net_s::Step &st, int precision) {
+  if (precision != QTNH_PREC_SPLIT) return QTNH_PREC_FP64;
+  return ((st.M + 127) / 128) * ((2 * st.N + 127) / 128) >= kSplitMinTiles ? QTNH_PREC_SPLIT : QTNH_PREC_FP64;
+}
 
 // Route launch_contract takes for a step (default placement) at a precision.
 ContractRoute step_route(const qtnh_net_s &net, const qtnh_net_s::Step &st, int precision) {
@@ -322,9 +333,12 @@ qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
     for (uint64_t d : B.dims) st.binary = st.binary && d == 2;
     st.binary = st.binary && A.dims.size() <= kGMaxBits && B.dims.size() <= kGMaxBits && r.dims.size() <= kGMaxBits;
     // Route and workspace, as launch_contract will choose them.
-    const ContractRoute R = step_route(net, st, net.precision);
+    const ContractRoute R = step_route(net, st, QTNH_PREC_FP64);
     st.direct = R.direct;
-    st.ws = R.ws_bytes;
+    st.M = R.M;
+    st.N = R.N;
+    st.ws = step_precision(st, net.precision) == QTNH_PREC_SPLIT ? step_route(net, st, QTNH_PREC_SPLIT).ws_bytes
+                                                                  : R.ws_bytes;
     net.ws_bytes = std::max(net.ws_bytes, st.ws);
     // Arena: the result is placed before its inputs are released (no aliasing).
     r.offset = arena.alloc(r.size);
@@ -383,7 +397,7 @@ qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t 
   default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
   return launch_contract(ctx, node_ptr(net, block, st.a), (int)A.dims.size(), A.dims.data(), node_ptr(net, block, st.b),
                          (int)B.dims.size(), B.dims.data(), nw, st.wires.data(), r1.data(), r2.data(),
-                         node_ptr(net, block, st.c), precision, nullptr, ws, ws_bytes);
+                         node_ptr(net, block, st.c), step_precision(st, precision), nullptr, ws, ws_bytes);
 }
 
 GDesc gdesc_template(const qtnh_net_s &net, size_t s) {
@@ -739,7 +753,7 @@ qtnh_status qtnh_net_set_precision(qtnh_net_t net, int precision) {
   // The order and the routes stay; the GEMM steps' workspace follows the precision.
   net->ws_bytes = 0;
   for (auto &st : net->steps) {
-    if (!st.direct) st.ws = step_route(*net, st, precision).ws_bytes;
+    if (!st.direct) st.ws = step_route(*net, st, step_precision(st, precision)).ws_bytes;
     net->ws_bytes = std::max(net->ws_bytes, st.ws);
   }
   return QTNH_OK;

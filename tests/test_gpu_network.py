@@ -140,8 +140,9 @@ def test_split_precision_networks():
     """GEMM-route steps in split precision: one-at-a-time and batched runs agree
     bit for bit (the same deterministic kernels), the amplitudes stay within
     1e-9 relative of the FP64 ones (each split GEMM is within about 1e-10
-    normwise; these networks have 2 to 10 such steps), and setting FP64 back
-    restores the FP64 results bit for bit."""
+    normwise), and setting FP64 back restores the FP64 results bit for bit.
+    Only steps whose result fills the GPU take the split GEMM: the two smaller
+    networks have none and stay exact, the 5x5 depth-14 one has some."""
     from qtnh_amd import _lib
     nets = []
     for i, (r, c, d) in enumerate([(4, 4, 12), (5, 5, 10), (5, 5, 14)]):
@@ -159,7 +160,8 @@ def test_split_precision_networks():
         errs.append(np.abs(s - f).max() / np.abs(f).max())
     print("split relative errors", errs)
     assert max(errs) <= 1e-9
-    assert max(errs) > 0  # the split GEMM ran
+    assert errs[0] == 0 and errs[1] == 0
+    assert errs[2] > 0  # the split GEMM ran
     for n in nets:
         n.set_precision(_lib.PREC_FP64)
     back = [n.run() for n in nets]
@@ -171,13 +173,14 @@ def test_prepared_batch_keeps_its_precision():
     """A batch prepared in FP64 (eager and graph) still gives the FP64 results
     after its networks are switched to split precision."""
     from qtnh_amd import _lib
-    circ = ops.Circuit(25).grid(5, 5, 10, 507)
-    net = ops.Network.amplitude(circ, np.random.default_rng(7).integers(0, 2, 25)).plan(7)
+    circ = ops.Circuit(25).grid(5, 5, 14, 508)
+    net = ops.Network.amplitude(circ, np.random.default_rng(8).integers(0, 2, 25)).plan(8)
     want = net.run()
     eager = ops.NetworkBatch([net], use_graph=False)
     graph = ops.NetworkBatch([net], use_graph=True)
     net.set_precision(_lib.PREC_SPLIT)
     try:
+        assert not same_bits(net.run(), want)  # this network has split-precision steps
         assert same_bits(eager.run()[0], want)
         assert same_bits(graph.run()[0], want)
     finally:
