@@ -308,6 +308,27 @@ def test_contract_small_gate_outputs_last_bitexact(da, db, wires):
     assert same_bits(host(out), ref)
 
 
+_ROW_SHAPES = [(nb, kb) for nb in range(1, 5) for kb in range(1, 6 - nb)]
+
+
+@pytest.mark.parametrize("nb,kb", _ROW_SHAPES)
+def test_contract_row_kernel_every_shape_bitexact(nb, kb):
+    """Every (result bits, term bits) shape of the row kernel: staged reads
+    (up to 8 terms) or direct ones (16), one or two rows per thread, one or two
+    result passes; contracted axes scattered over a rank-16 state (>= 12 row bits)."""
+    rng = np.random.default_rng(100 * nb + kb)
+    da = [2] * 16
+    db = [2] * (kb + nb)
+    axes = rng.choice(16, size=kb, replace=False)
+    wires = [(int(a), i) for i, a in enumerate(axes)]
+    a = rnd(1 << 16, 31 + nb)
+    b = rnd(1 << (kb + nb), 41 + kb)
+    out, dc, _ = ops.contract(dev(a), da, dev(b), db, wires)
+    ref, rdc = oracle.contract(a, da, b, db, wires)
+    assert dc == rdc
+    assert same_bits(host(out), ref)
+
+
 def test_contract_small_gate_custom_placement_bitexact():
     """Gate outputs placed below the state's open axes too (not only last)."""
     da, db = [2] * 15, [2, 2, 2, 2]
