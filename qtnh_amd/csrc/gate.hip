@@ -167,6 +167,36 @@ __global__ void __launch_bounds__(kGateThreads) gate_lanes_kernel(c128 *__restri
   constexpr int U = KH >= 2 ? 1 : 4 >> KH;
   double2_t *p2 = reinterpret_cast<double2_t *>(psi);
   const int lane = threadIdx.x & 63;
+  // The output index o of high combination h depends only on the lane, so the
+  // coefficients G[i][o] a thread uses are fixed: they are read once instead of
+  // per element from the argument block (a per-lane vector-memory load whose
+  // wait would also wait for the stream's loads and stores in flight).  Held in
+  // registers when all gate bits are lane bits (measured faster there: the
+  // exchanges already load the LDS pipe), in LDS otherwise (faster with high
+  // gate bits, e.g. 77% -> 79% of HBM peak for a 2-qubit gate with one).
+  constexpr bool kRegCoef = KH == 0;
+  int outs[H];
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    int o = 0;
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int bit = args.jh[j] >= 0 ? ((h >> args.jh[j]) & 1) : ((lane >> args.jpos[j]) & 1);
+      o |= bit << (KB - 1 - j);
+    }
+    outs[h] = o;
+  }
+  double2_t greg[kRegCoef ? H : 1][kRegCoef ? K : 1];
+  __shared__ double2_t gco[kRegCoef ? 1 : K * K];
+  if constexpr (kRegCoef) {
+#pragma unroll
+    for (int h = 0; h < H; ++h)
+#pragma unroll
+      for (int i = 0; i < K; ++i) greg[h][i] = double2_t{args.g[i * K + outs[h]].re, args.g[i * K + outs[h]].im};
+  } else {
+    for (int i = threadIdx.x; i < K * K; i += kGateThreads) gco[i] = double2_t{args.g[i].re, args.g[i].im};
+    __syncthreads();
+  }
   const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
   // Every lane of a wave takes part in the shuffles: the loop bounds are
   // wave-uniform (n_rows is a multiple of 64 and so is the stride).
@@ -200,18 +230,12 @@ __global__ void __launch_bounds__(kGateThreads) gate_lanes_kernel(c128 *__restri
       }
 #pragma unroll
       for (int h = 0; h < H; ++h) {
-        // This lane's output index for high combination h.
-        int o = 0;
-#pragma unroll
-        for (int j = 0; j < KB; ++j) {
-          const int bit = args.jh[j] >= 0 ? ((h >> args.jh[j]) & 1) : ((lane >> args.jpos[j]) & 1);
-          o |= bit << (KB - 1 - j);
-        }
         double ar = 0.0, ai = 0.0;
 #pragma unroll
         for (int i = 0; i < K; ++i) {
           const c128 xi = {in[i].x, in[i].y};
-          const c128 pr = cmul(xi, args.g[i * K + o]);
+          const double2_t gi = kRegCoef ? greg[kRegCoef ? h : 0][kRegCoef ? i : 0] : gco[i * K + outs[h]];
+          const c128 pr = cmul(xi, c128{gi.x, gi.y});
           ar = ar + pr.re;
           ai = ai + pr.im;
         }
