@@ -303,8 +303,9 @@ void qtnh_tune_permute_auto(int on);
 void qtnh_tune_zgemm(int bn);
 /* Number of 8-bit slices (1..8, default 5) of QTNH_PREC_SPLIT contractions. */
 void qtnh_tune_split(int slices);
-/* Most workgroups (256..2^22, default 2^20) of the streaming gate and
- * diagonal kernels, which loop over the rest. */
+/* Most workgroups (256..2^22, default 2^20) of the streaming gate kernel for
+ * gates on high bits, which loops over the rest.  (The diagonal kernel and the
+ * kernels for gates on the lowest 6 bits size their grids to the state.) */
 void qtnh_tune_stream(int max_blocks);
 /* Tile bits (3..13, default 12) of fused circuit runs, and how many of the
  * lowest state bits every tile keeps (0..8, default 4; 2^low x 16 bytes are

@@ -25,6 +25,8 @@ constexpr int kGateThreads = 256;
 // Most workgroups of the streaming gate and diagonal kernels (they loop over
 // the rest): enough to keep many requests in flight on every CU.
 unsigned g_stream_blocks = 1u << 20;
+// Grid limit of the kernels sized to one element (or row) per thread.
+constexpr unsigned kMaxStreamBlocks = 1u << 30;
 
 __device__ inline c128 cmul(c128 a, c128 b) {
   c128 r;
@@ -285,8 +287,13 @@ qtnh_status launch_gate_lanes(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *
     args.src_lbits[i] = (uint8_t)lb;
     args.src_h[i] = (uint8_t)h;
   }
+  // Grid (measured at 2^28 and 2^30 elements): 1- and 2-qubit gates do best
+  // with one row per thread, so that the workgroups in flight cover one
+  // contiguous stretch of the state; 3-qubit gates (3 waves per SIMD or fewer)
+  // need each thread's U rows in flight together (one pass).
   const uint64_t rows = n >> KH;
-  unsigned blocks = grid_for(rows, kGateThreads, g_stream_blocks);
+  constexpr uint64_t kRowsPerThread = KB <= 2 ? 1 : (KH >= 2 ? 1 : 4 >> KH);
+  unsigned blocks = grid_for((rows + kRowsPerThread - 1) / kRowsPerThread, kGateThreads, kMaxStreamBlocks);
   hipLaunchKernelGGL((gate_lanes_kernel<KB, KH>), dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, rows);
   return check_hip(ctx, hipGetLastError(), "gate launch");
 }
@@ -397,7 +404,10 @@ qtnh_status launch_diag(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *d
     for (int m = 0; m < k; ++m)
       for (int t = L.width[axes[m]] - 1; t >= 0; --t) args.bitpos[kb++] = L.shift[axes[m]] + t;
     args.kb = kb;
-    unsigned blocks = grid_for((n + 3) / 4, kGateThreads, g_stream_blocks);
+    // One element per thread: the workgroups in flight then cover one
+    // contiguous stretch of the state (81% of HBM peak at 2^28 and 2^30
+    // elements, against 68-76% with four elements per thread 2^26+ apart).
+    unsigned blocks = grid_for(n, kGateThreads, kMaxStreamBlocks);
     hipLaunchKernelGGL(diag_bits_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, state, args, n);
     return check_hip(ctx, hipGetLastError(), "diag launch");
   }
