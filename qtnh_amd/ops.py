@@ -243,7 +243,7 @@ def tune_split(slices=5):
 
 
 def tune_stream(max_blocks=1 << 20):
-    """Most workgroups of the streaming gate and diagonal kernels (process-wide)."""
+    """Most workgroups of the streaming kernel for gates on high bits (process-wide)."""
     lib().qtnh_tune_stream(max_blocks)
 
 
