@@ -177,12 +177,16 @@ def bench_c2b(torch, ops, rank, seed):
 def bench_c1(torch, ops, _lib):
     """QFT-12 (reference examples/qft.cpp, tests/gen/qft.hpp): latency of one
     circuit application to a resident 4096-amplitude state, one launch per
-    gate vs fused; the two results are compared bitwise."""
+    gate vs fused (default tile, then the tile qtnh_circuit_tune picks for
+    this circuit); the results are compared bitwise."""
     n = 12
     c = ops.Circuit(n).qft()
     out = {"qubits": n, "gates": c.info()[1]}
     res = {}
-    for name, mode in (("per_gate", _lib.RUN_PER_GATE), ("fused", _lib.RUN_FUSED)):
+    for name, mode in (("per_gate", _lib.RUN_PER_GATE), ("fused", _lib.RUN_FUSED), ("fused_tuned", _lib.RUN_FUSED)):
+        if name == "fused_tuned":
+            tb, lb, _ = c.tune()
+            out["tuned_tile"] = {"tile_bits": tb, "low_bits": lb, "runs": c.info()[2]}
         s = ops.fill_(ops.empty(1 << n), 7)
         ops.scale_(s, 1 / math.sqrt(ops.norm2(s)))
         x0 = s.clone()
@@ -200,7 +204,9 @@ def bench_c1(torch, ops, _lib):
             lat.append((time.perf_counter() - t0) * 1e6)
         res[name] = s
         out[name] = {"p50_us": round(_pct(lat, 50), 1), "p99_us": round(_pct(lat, 99), 1)}
-    out["bitwise_equal"] = bool(torch.equal(torch.view_as_real(res["per_gate"]), torch.view_as_real(res["fused"])))
+    pg = torch.view_as_real(res["per_gate"])
+    out["bitwise_equal"] = bool(torch.equal(pg, torch.view_as_real(res["fused"])) and
+                                torch.equal(pg, torch.view_as_real(res["fused_tuned"])))
     return out
 
 

@@ -167,7 +167,8 @@ qtnh_status qtnh_circuit_random(qtnh_circuit_t c, int depth, uint64_t seed);
  * CZ on grid neighbours, cycling horizontal-even, horizontal-odd,
  * vertical-even, vertical-odd couplers layer by layer. */
 qtnh_status qtnh_circuit_grid(qtnh_circuit_t c, int rows, int cols, int depth, uint64_t seed);
-/* Number of qubits, ops, and runs under the current tile size (any may be NULL). */
+/* Number of qubits, ops, and runs under the circuit's tuned tile (qtnh_circuit_tune), else the
+ * process default tile (any may be NULL). */
 qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, uint64_t *n_runs);
 /* Reads op i back: kind (0 gate, 1 diagonal), k, qubits[k], coefficients. */
 qtnh_status qtnh_circuit_op(qtnh_circuit_t c, uint64_t i, int *kind, int *k, uint16_t *qubits, double *coef);
@@ -277,6 +278,15 @@ qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *queries, qtnh_q
  * run on this context (results do not depend on it).  Outputs may be NULL:
  * the chosen tile bits, low bits and the best time (ms). */
 qtnh_status qtnh_autotune_circuit(qtnh_ctx_t ctx, int n_qubits, int *tile_bits, int *low_bits, double *ms);
+/* Times this circuit's own fused runs at several tile shapes (12 bits down to
+ * 7) on `scratch`, a device buffer of 2^n complex128 elements whose contents
+ * it overwrites, and keeps the fastest for every later fused run of the
+ * circuit (any state, any context; results do not depend on it).  Worth it
+ * for circuits run many times: the best tile depends on the circuit (QFT-12:
+ * 8 bits, 65 against 77 us per run), not only on its size.  Outputs may be
+ * NULL: tile bits, low bits, time per run (us).  No reference counterpart. */
+qtnh_status qtnh_circuit_tune(qtnh_ctx_t ctx, qtnh_circuit_t c, void *scratch, int *tile_bits, int *low_bits,
+                              double *us);
 
 /* ---- queries on a state (no reference counterpart) ---- */
 /* out[j] = <psi| Z_{qubits[j]} |psi> = sum_x |psi_x|^2 (1 - 2 x_q), fixed

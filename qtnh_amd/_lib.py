@@ -95,6 +95,7 @@ SIGNATURES = {
     "qtnh_net_batch_run": [_p, _p, ctypes.POINTER(_dp)],
     "qtnh_net_batch_destroy": [_p],
     "qtnh_autotune_circuit": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _dp],
+    "qtnh_circuit_tune": [_p, _p, _p, ctypes.POINTER(_i), ctypes.POINTER(_i), _dp],
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_permute_auto": None, "qtnh_tune_zgemm": None,

@@ -853,6 +853,8 @@ struct qtnh_circuit_s {
   };
   int n = 0;
   std::vector<Op> ops;
+  // Tile shape chosen for this circuit by qtnh_circuit_tune (0: the context's).
+  int tuned_tb = 0, tuned_low = 0;
   // Compiled form (device), rebuilt when ops or the tile size change.
   int compiled_tb = -1;
   int device = -1;
@@ -1012,6 +1014,10 @@ void tile_shape(qtnh_ctx_t ctx, int n, int *tb, int *low) {
 qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
   int tb, low;
   tile_shape(ctx, c.n, &tb, &low);
+  if (c.tuned_tb > 0) {
+    tb = c.tuned_tb;
+    low = c.tuned_low;
+  }
   if (c.compiled_tb == tb * 16 + low && c.device == ctx->device && c.compiled_ops == c.ops.size()) return QTNH_OK;
   if (c.d_ops || c.d_coef) {
     qtnh_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "circuit recompile sync");
@@ -1528,7 +1534,7 @@ qtnh_status qtnh_circuit_info(qtnh_circuit_t c, int *n_qubits, uint64_t *n_ops, 
     qtnh_circuit_s tmp;
     tmp.n = c->n;
     tmp.ops = c->ops;
-    plan_runs(tmp, g_tile_bits, g_low_bits);
+    plan_runs(tmp, c->tuned_tb > 0 ? c->tuned_tb : g_tile_bits, c->tuned_tb > 0 ? c->tuned_low : g_low_bits);
     *n_runs = tmp.runs.size();
   }
   return QTNH_OK;
@@ -1584,6 +1590,50 @@ qtnh_status qtnh_circuit_run_batch(qtnh_ctx_t ctx, int count, const qtnh_circuit
   return launch_run_groups(ctx, descs, starts);
 }
 
+
+qtnh_status qtnh_circuit_tune(qtnh_ctx_t ctx, qtnh_circuit_t c, void *scratch, int *tile_bits, int *low_bits,
+                              double *us_out) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!c || !scratch) return set_error(ctx, QTNH_EINVAL, "circuit_tune: null argument");
+  if (c->ops.empty()) return QTNH_OK;
+  // Candidates: the default tile, then smaller tiles (more workgroups per run;
+  // they win for small states whose runs would otherwise occupy a few CUs).
+  const int cand[][2] = {{12, 4}, {11, 4}, {10, 2}, {9, 2}, {8, 2}, {7, 2}};
+  const int reps = c->n >= 20 ? 2 : 10;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  st = check_hip(ctx, hipEventCreate(&e0), "circuit_tune event");
+  if (!st) st = check_hip(ctx, hipEventCreate(&e1), "circuit_tune event");
+  double best = 1e30;
+  int btb = 0, blow = 0;
+  for (const auto &cd : cand) {
+    if (st) break;
+    if (cd[0] > std::max(c->n, 7)) continue;
+    c->tuned_tb = cd[0];
+    c->tuned_low = cd[1];
+    st = run_fused(ctx, *c, (c128 *)scratch);  // compile + warm up
+    float ms = 0;
+    if (!st) st = check_hip(ctx, hipEventRecord(e0, ctx->stream), "circuit_tune record");
+    for (int r = 0; r < reps && !st; ++r) st = run_fused(ctx, *c, (c128 *)scratch);
+    if (!st) st = check_hip(ctx, hipEventRecord(e1, ctx->stream), "circuit_tune record");
+    if (!st) st = check_hip(ctx, hipEventSynchronize(e1), "circuit_tune sync");
+    if (!st) st = check_hip(ctx, hipEventElapsedTime(&ms, e0, e1), "circuit_tune elapsed");
+    if (!st && ms / reps < best) {
+      best = ms / reps;
+      btb = cd[0];
+      blow = cd[1];
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  c->tuned_tb = st ? 0 : btb;
+  c->tuned_low = st ? 0 : blow;
+  if (st) return st;
+  if (tile_bits) *tile_bits = btb;
+  if (low_bits) *low_bits = blow;
+  if (us_out) *us_out = best * 1e3;
+  return QTNH_OK;
+}
 
 qtnh_status qtnh_autotune_circuit(qtnh_ctx_t ctx, int n_qubits, int *tile_bits, int *low_bits, double *ms_out) {
   qtnh_status st = enter_ctx(ctx);

@@ -351,6 +351,20 @@ class Circuit:
             out.append((kind.value, [int(v) for v in q[:k.value]], co[:size].copy()))
         return out
 
+    def tune(self, scratch=None):
+        """Pick this circuit's fused tile by timing its own runs on `scratch`
+        (a 2^n complex128 device tensor, overwritten; allocated if None).
+        Returns (tile_bits, low_bits, us_per_run); results do not depend on it."""
+        if scratch is None:
+            scratch = empty(1 << self.n)
+        if scratch.numel() != (1 << self.n):
+            raise ValueError("scratch has %d amplitudes, circuit needs %d" % (scratch.numel(), 1 << self.n))
+        ctx = context(scratch.device.index)
+        tb, lb, us = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+        check(lib().qtnh_circuit_tune(ctx, self._h, _ptr(scratch), ctypes.byref(tb), ctypes.byref(lb),
+                                      ctypes.byref(us)), ctx)
+        return tb.value, lb.value, us.value
+
     def run(self, state, mode=RUN_FUSED):
         if state.numel() != (1 << self.n):
             raise ValueError("state has %d amplitudes, circuit needs %d" % (state.numel(), 1 << self.n))

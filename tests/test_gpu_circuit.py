@@ -211,3 +211,17 @@ def test_large_random_circuit_keeps_norm_and_modes_agree():
     assert abs(ops.norm2(a) - 1) < 1e-12
     z = ops.expect_z(a, n, [0, n // 2, n - 1])
     assert all(abs(v) <= 1 for v in z)
+
+
+@pytest.mark.parametrize("kind,n", [("qft", 12), ("random", 12), ("random", 16)])
+def test_circuit_tune_keeps_results(kind, n):
+    """A circuit tuned to its own fastest tile gives the per-gate executor's bits."""
+    c = ops.Circuit(n).qft() if kind == "qft" else ops.Circuit(n).random(6, 3)
+    x = rnd_state(n, 40 + n)
+    want = dev(x)
+    c.run(want, _lib.RUN_PER_GATE)
+    tb, lb, us = c.tune()
+    assert 7 <= tb <= 12 and lb in (2, 4) and us > 0
+    got = dev(x)
+    c.run(got, _lib.RUN_FUSED)
+    assert same_bits(host(got), host(want))
