@@ -177,7 +177,8 @@ qtnh_status qtnh_circuit_op(qtnh_circuit_t c, uint64_t i, int *kind, int *k, uin
  * for each state buffer (up to 64 per circuit): a later fused run of the same
  * circuit on the same buffer only launches kernels, so it may be captured
  * into a HIP graph; the graph stays valid while the circuit is not modified
- * or destroyed.  A fused run that would upload (first run on a buffer, or a
+ * or destroyed (tuning calls and runs on other devices recompile the circuit
+ * but keep the device memory earlier graphs read until then).  A fused run that would upload (first run on a buffer, or a
  * circuit not yet compiled) returns QTNH_EUNSUPPORTED while the context's
  * stream is being captured. */
 qtnh_status qtnh_circuit_run(qtnh_ctx_t ctx, qtnh_circuit_t c, void *state, int mode);

@@ -878,6 +878,11 @@ struct qtnh_circuit_s {
     std::vector<size_t> starts;
   };
   std::map<const void *, DescCache> desc_cache;
+  // Device memory of earlier compiled forms of the same ops (another tile
+  // shape or device).  A graph captured from an earlier fused run may still
+  // read it, so it is freed only when ops are appended or the circuit is
+  // destroyed.
+  std::vector<void *> retired;
 };
 
 namespace qtnh_hip {
@@ -933,17 +938,25 @@ void plan_runs(qtnh_circuit_s &c, int tb, int low_bits) {
   close((uint32_t)c.ops.size());
 }
 
-void free_compiled(qtnh_circuit_s &c) {
+// Detaches the current compiled form; its device memory moves to `retired`.
+void retire_compiled(qtnh_circuit_s &c) {
   for (auto &kv : c.desc_cache)
-    if (kv.second.d) (void)hipFree(kv.second.d);
+    if (kv.second.d) c.retired.push_back(kv.second.d);
   c.desc_cache.clear();
-  if (c.d_ops) (void)hipFree(c.d_ops);
-  if (c.d_coef) (void)hipFree(c.d_coef);
-  if (c.d_phases) (void)hipFree(c.d_phases);
+  for (void *p : {(void *)c.d_ops, (void *)c.d_coef, (void *)c.d_phases})
+    if (p) c.retired.push_back(p);
   c.d_ops = nullptr;
   c.d_coef = nullptr;
   c.d_phases = nullptr;
   c.compiled_tb = -1;
+}
+
+// Frees the current and every retired compiled form (the caller has drained
+// the streams that used them).
+void free_compiled(qtnh_circuit_s &c) {
+  retire_compiled(c);
+  for (void *p : c.retired) (void)hipFree(p);
+  c.retired.clear();
 }
 
 // Register phases of a 12-bit-tile run: consecutive ops whose tile bits fit 4,
@@ -1020,9 +1033,15 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
   }
   if (c.compiled_tb == tb * 16 + low && c.device == ctx->device && c.compiled_ops == c.ops.size()) return QTNH_OK;
   if (c.d_ops || c.d_coef) {
-    qtnh_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "circuit recompile sync");
-    if (st) return st;
-    free_compiled(c);
+    if (c.compiled_ops == c.ops.size()) {
+      // Same ops, another tile shape or device: graphs captured from the
+      // earlier form stay valid, so its memory is kept.
+      retire_compiled(c);
+    } else {
+      qtnh_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "circuit recompile sync");
+      if (st) return st;
+      free_compiled(c);
+    }
   }
   plan_runs(c, tb, low);
   std::vector<TileOp> tops(std::max<size_t>(c.ops.size(), 1));
@@ -1097,7 +1116,14 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
                                        ctx->stream),
                    "circuit upload");
   if (st) {
-    free_compiled(c);
+    // Only this attempt's buffers: no launch has used them (retired forms stay).
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : {(void *)c.d_ops, (void *)c.d_coef, (void *)c.d_phases})
+      if (p) (void)hipFree(p);
+    c.d_ops = nullptr;
+    c.d_coef = nullptr;
+    c.d_phases = nullptr;
+    c.compiled_tb = -1;
     return st;
   }
   c.tops = tops;
@@ -1628,6 +1654,9 @@ qtnh_status qtnh_circuit_tune(qtnh_ctx_t ctx, qtnh_circuit_t c, void *scratch, i
   if (e1) (void)hipEventDestroy(e1);
   c->tuned_tb = st ? 0 : btb;
   c->tuned_low = st ? 0 : blow;
+  if (st) return st;
+  // Leave the winning shape compiled, so the next run launches without a recompile.
+  st = compile(ctx, *c);
   if (st) return st;
   if (tile_bits) *tile_bits = btb;
   if (low_bits) *low_bits = blow;

@@ -82,6 +82,18 @@ def _check_len(t, dims):
         raise ValueError("tensor has %d elements, dims %s need %d" % (t.numel(), list(dims), _size(dims)))
 
 
+def _check_out(out, x, dims):
+    """A caller-supplied result buffer must hold exactly prod(dims) complex128
+    elements, contiguously, on x's device: the kernels write all of it."""
+    if not isinstance(out, torch.Tensor) or out.dtype != torch.complex128:
+        raise TypeError("out must be a complex128 tensor")
+    _check_len(out, dims)
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous")
+    if out.device != x.device:
+        raise ValueError("out is on %s, the operands on %s" % (out.device, x.device))
+
+
 def empty(n, device=None):
     return torch.empty(int(n), dtype=torch.complex128, device=device or "cuda")
 
@@ -95,17 +107,23 @@ def fill_(x, seed, dist=_lib.DIST_NORMAL):
 def permute(x, dims, ptup, out=None):
     """Result axis ptup[i] = axis i of x (reference dense.cpp:152-169)."""
     _check_len(x, dims)
-    out = empty(x.numel(), x.device) if out is None else out
+    if len(ptup) != len(dims):
+        raise ValueError("ptup length differs from rank")
+    rdims = permuted_dims(dims, ptup)
+    if out is None:
+        out = empty(x.numel(), x.device)
+    else:
+        _check_out(out, x, rdims)
     ctx = context(x.device.index)
     d, dp = _u64(dims)
     p, pp = _u16(ptup)
-    if len(ptup) != len(dims):
-        raise ValueError("ptup length differs from rank")
     check(lib().qtnh_permute_c128(ctx, _ptr(x), _ptr(out), len(dims), dp, pp), ctx)
     return out
 
 
 def permuted_dims(dims, ptup):
+    if sorted(int(v) for v in ptup) != list(range(len(dims))):
+        raise ValueError("ptup %s is not a permutation of %d axes" % (list(ptup), len(dims)))
     nd = [0] * len(dims)
     for i, d in enumerate(dims):
         nd[ptup[i]] = d
@@ -193,7 +211,12 @@ def contract(A, dA, B, dB, wires, repl1=None, repl2=None, precision=PREC_FP64, o
     if repl1 is None or repl2 is None:
         repl1, repl2 = default_repls(len(dA), 0, len(dB), 0, wires)
     dC = contract_dims(dA, dB, wires, repl1, repl2)
-    out = empty(_size(dC), A.device) if out is None else out
+    if B.device != A.device:
+        raise ValueError("A is on %s, B on %s" % (A.device, B.device))
+    if out is None:
+        out = empty(_size(dC), A.device)
+    else:
+        _check_out(out, A, dC)
     ctx = context(A.device.index)
     a, ap = _u64(dA)
     b, bp = _u64(dB)

@@ -105,3 +105,27 @@ def test_contract_dims_and_validation():
     # duplicate wire endpoints are rejected
     with pytest.raises(_lib.QtnhError):
         ops.contract_dims([2, 2], [2, 2], [(0, 0), (0, 1)], [X, 0], [X, X])
+
+
+def test_result_buffers_are_checked_before_any_launch():
+    """An undersized, mistyped or non-contiguous out= is refused on the host
+    before the library is called (the kernels write every element of out)."""
+    import torch
+    dims = [2] * 6
+    x = torch.zeros(64, dtype=torch.complex128)
+    ptup = [5, 4, 3, 2, 1, 0]
+    with pytest.raises(ValueError):
+        ops.permute(x, dims, ptup, out=torch.zeros(63, dtype=torch.complex128))
+    with pytest.raises(TypeError):
+        ops.permute(x, dims, ptup, out=torch.zeros(64, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        ops.permute(x, dims, ptup, out=torch.zeros(128, dtype=torch.complex128)[::2])
+    with pytest.raises(ValueError):
+        ops.permute(x, dims, [0, 1, 2, 3, 4, 4])
+    a = torch.zeros(16, dtype=torch.complex128)
+    b = torch.zeros(16, dtype=torch.complex128)
+    wires = [(3, 0), (2, 1)]
+    with pytest.raises(ValueError):
+        ops.contract(a, [2] * 4, b, [2] * 4, wires, out=torch.zeros(15, dtype=torch.complex128))
+    with pytest.raises(ValueError):
+        ops.contract(a, [2] * 4, b, [2] * 4, wires, out=torch.zeros(17, dtype=torch.complex128))

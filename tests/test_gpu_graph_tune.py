@@ -155,3 +155,46 @@ def test_autotune_keeps_bits():
     c2.run(b)
     torch.cuda.synchronize()
     assert torch.equal(torch.view_as_real(a), torch.view_as_real(b))
+
+
+def test_captured_run_survives_recompiles():
+    """A graph captured from a fused run keeps replaying the same bits after
+    the circuit is recompiled for other tile shapes (a process-wide tile
+    change, the per-circuit tuner) and run on another context: the earlier
+    compiled form's device memory is kept until the circuit is destroyed."""
+    n = 16
+    c = ops.Circuit(n).random(4, 77)
+    x0 = ops.fill_(ops.empty(1 << n), 21)
+    ref = x0.clone()
+    snaps = []
+    for _ in range(4):
+        c.run(ref)
+        snaps.append(ref.clone())
+    s = x0.clone()
+    cs = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(cs):
+        c.run(s)
+    cs.synchronize()
+    assert torch.equal(torch.view_as_real(s), torch.view_as_real(snaps[0]))
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=cs):
+        c.run(s)
+    other = ops.fill_(ops.empty(1 << n), 5)
+    try:
+        ops.tune_circuit(10, 2)
+        c.run(other)  # recompiled for 10-bit tiles
+        tb, lb, us = c.tune(ops.empty(1 << n))  # six more compiles, the winner left compiled
+        assert us > 0
+        cs2 = torch.cuda.Stream()
+        with torch.cuda.stream(cs2):
+            c.run(other)  # another context (stream) on the same device
+        cs2.synchronize()
+    finally:
+        ops.tune_circuit(12, 4)
+    for k in range(1, 4):
+        g.replay()
+        ops.scale_(other, 0.5)
+        torch.cuda.synchronize()
+        assert torch.equal(torch.view_as_real(s), torch.view_as_real(snaps[k])), k
+    c.close()

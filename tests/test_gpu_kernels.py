@@ -173,9 +173,17 @@ def test_permute_roundtrip_rank24():
 
 
 def test_permute_rejects_bad_tuple():
+    """Refused by the Python front end, and by the C ABI itself."""
     x = ops.empty(8)
-    with pytest.raises(_lib.QtnhError):
+    y = ops.empty(8)
+    with pytest.raises(ValueError):
         ops.permute(x, [2, 2, 2], [0, 1, 1])
+    d, dp = ops._u64([2, 2, 2])
+    p, pp = ops._u16([0, 1, 1])
+    ctx = ops.context()
+    st = _lib.lib().qtnh_permute_c128(ctx, ops._ptr(x), ops._ptr(y), 3, dp, pp)
+    assert st == 1, st
+    assert b"permutation" in _lib.lib().qtnh_ctx_last_error(ctx)
 
 
 # ------------------------------------------------------------------ swap
