@@ -132,6 +132,25 @@ qtnh_status qtnh_contract_c128(qtnh_ctx_t ctx, const void *A, int rA, const uint
                                const uint16_t *repl1, const uint16_t *repl2, void *C, int precision,
                                double *err_out);
 
+/* Bytes of the static device scratch of qtnh_contract_c128_noalloc (per GPU). */
+#define QTNH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* qtnh_contract_c128 without any device allocation.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same
+ * arguments, routes, kernels and results as qtnh_contract_c128; the GEMM
+ * route's workspace (permuted operands, natural result, split-precision digit
+ * planes, split-K and depth-chunk partials) is a file-scope device array of
+ * QTNH_NOALLOC_SCRATCH_BYTES that every GPU holds until the process ends, its
+ * address looked up once per device by qtnh_ctx_create.  A contraction whose
+ * workspace exceeds it returns QTNH_EINVAL before any launch.  Calls that
+ * overlap in time on one GPU (other streams, other threads) share the array
+ * and race; calls ordered on one stream do not. */
+qtnh_status qtnh_contract_c128_noalloc(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B,
+                                       int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                                       const uint16_t *repl1, const uint16_t *repl2, void *C, int precision,
+                                       double *err_out);
+
 /* ---- reductions used for normalisation and checks ---- */
 /* *out = sum |x_i|^2 (device reduction, fixed order for a given n). */
 qtnh_status qtnh_norm2_c128(qtnh_ctx_t ctx, const void *x, uint64_t n, double *out);

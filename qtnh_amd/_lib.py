@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libqtnh_hip.so")
 QTNH_OK, QTNH_EINVAL, QTNH_ENOMEM, QTNH_EHIP, QTNH_EUNSUPPORTED = range(5)
 X = 0xFFFF
 PREC_FP64, PREC_SPLIT = 0, 1
+NOALLOC_SCRATCH_BYTES = 64 << 20  # QTNH_NOALLOC_SCRATCH_BYTES
 RUN_PER_GATE, RUN_FUSED = 0, 1
 QUERY_AMPLITUDE, QUERY_EXPECT_Z = 0, 1
 SERVE_SEQUENTIAL, SERVE_BATCHED = 0, 1
@@ -57,6 +58,7 @@ SIGNATURES = {
     "qtnh_gate_apply_c128": [_p, _p, _i, _u64p, _i, _u16p, _dp, _i],
     "qtnh_diag_apply_c128": [_p, _p, _i, _u64p, _i, _u16p, _dp],
     "qtnh_contract_c128": [_p, _p, _i, _u64p, _p, _i, _u64p, _i, _u16p, _u16p, _u16p, _p, _i, _dp],
+    "qtnh_contract_c128_noalloc": [_p, _p, _i, _u64p, _p, _i, _u64p, _i, _u16p, _u16p, _u16p, _p, _i, _dp],
     "qtnh_norm2_c128": [_p, _p, _u64, _dp],
     "qtnh_scale_c128": [_p, _p, _u64, ctypes.c_double, ctypes.c_double],
     "qtnh_tune_permute": [_i, _i, _i, _i, _i, _i],

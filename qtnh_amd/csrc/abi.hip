@@ -235,6 +235,28 @@ qtnh_status contract_dims(int rA, const uint64_t *dA, int rB, const uint64_t *dB
   return QTNH_OK;
 }
 
+qtnh_status check_contract_args(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B, int rB,
+                                const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
+                                const uint16_t *repl2, const void *C, int precision, uint64_t *nC_out) {
+  *nC_out = 0;
+  if ((rA && !dA) || (rB && !dB) || (nw && !wires) || (rA && !repl1) || (rB && !repl2))
+    return set_error(ctx, QTNH_EINVAL, "contract: null argument");
+  if (precision != QTNH_PREC_FP64 && precision != QTNH_PREC_SPLIT)
+    return set_error(ctx, QTNH_EINVAL, "contract: bad precision %d", precision);
+  int rC;
+  uint64_t dC[kMaxRank];
+  if (contract_dims(rA, dA, rB, dB, nw, wires, repl1, repl2, &rC, dC) != QTNH_OK)
+    return set_error(ctx, QTNH_EINVAL, "contract: incompatible wires or placement");
+  uint64_t nA, nB, nC;
+  if (!checked_size(rA, dA, &nA) || !checked_size(rB, dB, &nB) || !checked_size(rC, dC, &nC))
+    return set_error(ctx, QTNH_EINVAL, "contract: size overflows");
+  if (!nC) return QTNH_OK;
+  if (!C || (nA && !A) || (nB && !B)) return set_error(ctx, QTNH_EINVAL, "contract: null buffer");
+  if (C == A || C == B) return set_error(ctx, QTNH_EINVAL, "contract: output aliases an input");
+  *nC_out = nC;
+  return QTNH_OK;
+}
+
 }  // namespace qtnh_hip
 
 using namespace qtnh_hip;
@@ -296,6 +318,8 @@ qtnh_status qtnh_ctx_create(int device, void *stream, qtnh_ctx_t *out) {
   if (st == QTNH_OK) ctx->stream = (hipStream_t)stream;
   if (st == QTNH_OK) st = check_hip(ctx, hipHostMalloc((void **)&ctx->host_scalar, 64), "hipHostMalloc");
   if (st == QTNH_OK) st = check_hip(ctx, hipMalloc((void **)&ctx->dev_scalar, 2048 * sizeof(double)), "hipMalloc scalar");
+  // Device address of the no-allocation entry point's scratch array (once per device).
+  if (st == QTNH_OK) st = noalloc_setup(ctx);
   if (st != QTNH_OK) {
     qtnh_ctx_destroy(ctx);
     return st;
@@ -481,26 +505,16 @@ qtnh_status qtnh_diag_apply_c128(qtnh_ctx_t ctx, void *state, int rank, const ui
   return launch_diag(ctx, (c128 *)state, rank, dims, k, axes, d);
 }
 
+
 qtnh_status qtnh_contract_c128(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B, int rB,
                                const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
                                const uint16_t *repl2, void *C, int precision, double *err_out) {
   if (need_ctx(ctx)) return QTNH_EINVAL;
   if (err_out) *err_out = 0.0;
-  if ((rA && !dA) || (rB && !dB) || (nw && !wires) || (rA && !repl1) || (rB && !repl2))
-    return set_error(ctx, QTNH_EINVAL, "contract: null argument");
-  if (precision != QTNH_PREC_FP64 && precision != QTNH_PREC_SPLIT)
-    return set_error(ctx, QTNH_EINVAL, "contract: bad precision %d", precision);
-  int rC;
-  uint64_t dC[kMaxRank];
-  if (contract_dims(rA, dA, rB, dB, nw, wires, repl1, repl2, &rC, dC) != QTNH_OK)
-    return set_error(ctx, QTNH_EINVAL, "contract: incompatible wires or placement");
-  uint64_t nA, nB, nC;
-  if (!checked_size(rA, dA, &nA) || !checked_size(rB, dB, &nB) || !checked_size(rC, dC, &nC))
-    return set_error(ctx, QTNH_EINVAL, "contract: size overflows");
-  if (!nC) return QTNH_OK;
-  if (!C || (nA && !A) || (nB && !B)) return set_error(ctx, QTNH_EINVAL, "contract: null buffer");
-  if (C == A || C == B) return set_error(ctx, QTNH_EINVAL, "contract: output aliases an input");
-  qtnh_status st = enter(ctx);
+  uint64_t nC = 0;
+  qtnh_status st = check_contract_args(ctx, A, rA, dA, B, rB, dB, nw, wires, repl1, repl2, C, precision, &nC);
+  if (st || !nC) return st;
+  st = enter(ctx);
   if (st) return st;
   return launch_contract(ctx, (const c128 *)A, rA, dA, (const c128 *)B, rB, dB, nw, wires, repl1, repl2, (c128 *)C,
                          precision, err_out);

@@ -594,6 +594,56 @@ static uint64_t splitk_chunks(uint64_t MN, uint64_t K) {
   return std::max<uint64_t>(1, std::min(want, most));
 }
 
+// Operand and result layouts of the GEMM route and its workspace: A' = A
+// permuted to [open1, closed]; B' = B permuted to [closed, open2] (FP64 GEMM)
+// or [open2, closed] (split precision and split-K, which read B by columns);
+// the natural result [open1, open2] is permuted to the placement.  Operands
+// already in those layouts are used in place.  Workspace: A', B', natural C,
+// then split scratch, split-K partials or GEMM depth-chunk partials.
+struct GemmLayout {
+  bool splitk, split;
+  std::vector<uint16_t> pA, pB, pC;
+  bool idA = true, idB = true, idC = true;
+  uint64_t kchunks, zchunks, sbytes, bytes;
+};
+
+static uint64_t pad_c128(uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; }
+
+static GemmLayout gemm_layout(const Plan &P, int precision) {
+  GemmLayout L;
+  L.splitk = use_splitk(P);
+  L.split = precision == QTNH_PREC_SPLIT && !L.splitk;
+  L.pA.resize(P.rA);
+  L.pB.resize(P.rB);
+  L.pC.resize(P.rC);
+  int q = 0;
+  for (int i : P.open1) L.pA[i] = (uint16_t)q++;
+  for (int i : P.cA) L.pA[i] = (uint16_t)q++;
+  q = 0;
+  if (L.split || L.splitk) {
+    for (int i : P.open2) L.pB[i] = (uint16_t)q++;
+    for (int i : P.cB) L.pB[i] = (uint16_t)q++;
+  } else {
+    for (int i : P.cB) L.pB[i] = (uint16_t)q++;
+    for (int i : P.open2) L.pB[i] = (uint16_t)q++;
+  }
+  for (int i = 0; i < P.rA; ++i) L.idA = L.idA && L.pA[i] == i;
+  for (int i = 0; i < P.rB; ++i) L.idB = L.idB && L.pB[i] == i;
+  for (int i = 0; i < P.rC; ++i) {
+    L.pC[i] = (uint16_t)P.pos[i];
+    L.idC = L.idC && P.pos[i] == i;
+  }
+  L.kchunks = L.splitk ? splitk_chunks(P.M * P.N, P.K) : 0;
+  L.zchunks = (L.split || L.splitk) ? 1 : zgemm_chunks(P.M, P.N, P.K);
+  L.sbytes = L.split         ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull)
+             : L.splitk      ? pad_c128(P.M * P.N * L.kchunks)
+             : L.zchunks > 1 ? pad_c128(P.M * P.N * L.zchunks)
+                             : 0;
+  L.bytes = (L.idA ? 0 : pad_c128(P.M * P.K)) + (L.idB ? 0 : pad_c128(P.K * P.N)) + (L.idC ? 0 : pad_c128(P.nC)) +
+            L.sbytes;
+  return L;
+}
+
 ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
                              const uint16_t *r1, const uint16_t *r2, int precision) {
   Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
@@ -602,19 +652,7 @@ ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t 
   R.N = P.N;
   R.K = P.K;
   R.direct = use_direct(P);
-  R.ws_bytes = 0;
-  if (!R.direct) {
-    // Same layout as launch_contract below: permuted A, permuted B, natural C,
-    // then split scratch, split-K partials or GEMM depth-chunk partials.
-    auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
-    R.ws_bytes = pad(P.M * P.K) + pad(P.K * P.N) + pad(P.nC);
-    if (use_splitk(P))
-      R.ws_bytes += pad(P.M * P.N * splitk_chunks(P.M * P.N, P.K));
-    else if (precision == QTNH_PREC_SPLIT)
-      R.ws_bytes += (split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull;
-    else if (zgemm_chunks(P.M, P.N, P.K) > 1)
-      R.ws_bytes += pad(P.M * P.N * zgemm_chunks(P.M, P.N, P.K));
-  }
+  R.ws_bytes = R.direct ? 0 : gemm_layout(P, precision).bytes;
   return R;
 }
 
@@ -627,40 +665,8 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     if (err_out) *err_out = 0.0;
     return launch_contract_direct(ctx, A, rA, dA, B, rB, dB, nw, wires, r1, r2, C);
   }
-  const bool splitk = use_splitk(P);
-  const bool split = precision == QTNH_PREC_SPLIT && !splitk;
-
-  // GEMM route.  A' = A permuted to [open1, closed]; B' = B permuted to
-  // [closed, open2] (FP64 GEMM) or [open2, closed] (split precision and
-  // split-K, which read B by columns).
-  std::vector<uint16_t> pA(rA), pB(rB), pC(P.rC);
-  int q = 0;
-  for (int i : P.open1) pA[i] = (uint16_t)q++;
-  for (int i : P.cA) pA[i] = (uint16_t)q++;
-  q = 0;
-  if (split || splitk) {
-    for (int i : P.open2) pB[i] = (uint16_t)q++;
-    for (int i : P.cB) pB[i] = (uint16_t)q++;
-  } else {
-    for (int i : P.cB) pB[i] = (uint16_t)q++;
-    for (int i : P.open2) pB[i] = (uint16_t)q++;
-  }
-  bool idA = true, idB = true, idC = true;
-  for (int i = 0; i < rA; ++i) idA = idA && pA[i] == i;
-  for (int i = 0; i < rB; ++i) idB = idB && pB[i] == i;
-  for (int i = 0; i < P.rC; ++i) {
-    pC[i] = (uint16_t)P.pos[i];
-    idC = idC && P.pos[i] == i;
-  }
-  const uint64_t nA = P.M * P.K, nB = P.K * P.N;
-  auto pad = [](uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; };
-  const uint64_t kchunks = splitk ? splitk_chunks(P.M * P.N, P.K) : 0;
-  const uint64_t zchunks = (split || splitk) ? 1 : zgemm_chunks(P.M, P.N, P.K);
-  const uint64_t sbytes = split         ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull)
-                          : splitk      ? pad(P.M * P.N * kchunks)
-                          : zchunks > 1 ? pad(P.M * P.N * zchunks)
-                                        : 0;
-  const uint64_t bytes = (idA ? 0 : pad(nA)) + (idB ? 0 : pad(nB)) + (idC ? 0 : pad(P.nC)) + sbytes;
+  const GemmLayout L = gemm_layout(P, precision);
+  const uint64_t bytes = L.bytes;
   char *ws = nullptr;
   if (bytes && ext_ws) {
     // A caller-provided region (batches, captured graphs): never reallocated here.
@@ -676,41 +682,42 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
   const c128 *Ap = A, *Bp = B;
   c128 *Cn = C;
   uint64_t off = 0;
-  if (!idA) {
+  if (!L.idA) {
     c128 *t = (c128 *)(ws + off);
-    off += pad(nA);
-    qtnh_status st = launch_permute(ctx, A, t, rA, dA, pA.data());
+    off += pad_c128(P.M * P.K);
+    qtnh_status st = launch_permute(ctx, A, t, rA, dA, L.pA.data());
     if (st) return st;
     Ap = t;
   }
-  if (!idB) {
+  if (!L.idB) {
     c128 *t = (c128 *)(ws + off);
-    off += pad(nB);
-    qtnh_status st = launch_permute(ctx, B, t, rB, dB, pB.data());
+    off += pad_c128(P.K * P.N);
+    qtnh_status st = launch_permute(ctx, B, t, rB, dB, L.pB.data());
     if (st) return st;
     Bp = t;
   }
-  if (!idC) {
+  if (!L.idC) {
     Cn = (c128 *)(ws + off);
-    off += pad(P.nC);
+    off += pad_c128(P.nC);
   }
   qtnh_status st;
-  if (splitk) {
-    st = launch_zgemm_splitk(ctx, Ap, Bp, Cn, P.M, P.N, P.K, kchunks, (c128 *)(ws + off));
+  if (L.splitk) {
+    st = launch_zgemm_splitk(ctx, Ap, Bp, Cn, P.M, P.N, P.K, L.kchunks, (c128 *)(ws + off));
     if (err_out) *err_out = 0.0;
-  } else if (split) {
-    st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + off, sbytes, err_out);
+  } else if (L.split) {
+    st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + off, L.sbytes, err_out);
   } else {
-    st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N, zchunks > 1 ? (c128 *)(ws + off) : nullptr);
+    st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N,
+                          L.zchunks > 1 ? (c128 *)(ws + off) : nullptr);
     if (err_out) *err_out = 0.0;
   }
   if (st) return st;
-  if (!idC) {
+  if (!L.idC) {
     // Natural result dims: open1 dims then open2 dims.
     std::vector<uint64_t> dn;
     for (int i : P.open1) dn.push_back(dA[i]);
     for (int i : P.open2) dn.push_back(dB[i]);
-    st = launch_permute(ctx, Cn, C, P.rC, dn.data(), pC.data());
+    st = launch_permute(ctx, Cn, C, P.rC, dn.data(), L.pC.data());
     if (st) return st;
   }
   return QTNH_OK;

@@ -113,6 +113,14 @@ ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t 
 qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
                                    const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
                                    const uint16_t *repl2, c128 *C);
+// Argument checks of qtnh_contract_c128 and its no-allocation variant; *nC
+// receives the result's element count (0: nothing to do).
+qtnh_status check_contract_args(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B, int rB,
+                                const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
+                                const uint16_t *repl2, const void *C, int precision, uint64_t *nC);
+// Looks up the device address of the no-allocation scratch array on the
+// context's device (current), once per device (noalloc.hip).
+qtnh_status noalloc_setup(qtnh_ctx_t ctx);
 qtnh_status launch_norm2(qtnh_ctx_t ctx, const c128 *x, uint64_t n, double *out);
 qtnh_status launch_scale(qtnh_ctx_t ctx, c128 *x, uint64_t n, double re, double im);
 // Zero fill with a kernel (capturable in graphs, unlike a memset node).
