@@ -216,6 +216,17 @@ qtnh_status qtnh_net_destroy(qtnh_net_t net);
 /* Adds a leaf tensor (host data, row-major complex); *id (may be NULL) gets its node id. */
 qtnh_status qtnh_net_add_tensor(qtnh_net_t net, int rank, const uint64_t *dims, const int64_t *bonds,
                                 const double *data, int *id);
+/* The best of `tries` greedy plans (seeds seed, seed + 1, ...): the fewest
+ * flops (sum of 8 M N K) among those whose intermediates stay within
+ * max_rank; *chosen_seed (may be NULL) gets its seed.  QTNH_EUNSUPPORTED when
+ * no try fits the cap. */
+qtnh_status qtnh_net_plan_best(qtnh_net_t net, uint64_t seed, int max_rank, int tries, uint64_t *chosen_seed);
+/* Adds a leaf whose elements stay in caller device memory (row-major complex,
+ * read in place by every run; it must stay valid and unchanged until the runs
+ * that read it have finished).  bonds may be NULL when the network is planned
+ * with qtnh_net_plan_steps, which does not use labels. */
+qtnh_status qtnh_net_add_tensor_device(qtnh_net_t net, int rank, const uint64_t *dims, const int64_t *bonds,
+                                       const void *data, int *id);
 /* Builds the amplitude network <bits| C |0...0> of a circuit: |0> vectors,
  * one tensor per op (axes: inputs, then outputs; diagonals as dense tensors),
  * and <bits_q| vectors (bits NULL: all zeros).  The network must be empty. */
@@ -223,6 +234,25 @@ qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_
 /* Greedy pairwise order (seeded ties) with fixed arena offsets for every
  * intermediate.  QTNH_EUNSUPPORTED when an intermediate would exceed max_rank. */
 qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank);
+/* One step of a caller-given order: contract node a with node b over nw wires
+ * (axis of a, axis of b) and place the result by repl1 / repl2 (the result
+ * position of each axis of a and of b, QTNH_X for wired axes; both NULL: the
+ * default placement, open axes of a then of b). */
+typedef struct {
+  int a, b;
+  int nw;
+  const uint16_t *wires;
+  const uint16_t *repl1;
+  const uint16_t *repl2;
+} qtnh_net_step_spec;
+/* Plans the network in a caller-given order instead of the greedy one: leaves
+ * are nodes 0..L-1 and the result of step s is node L+s; there are L-1 steps,
+ * each consuming two live nodes, so the last step's result is the network's.
+ * Intermediates get fixed arena offsets as with qtnh_net_plan.  The order of
+ * TensorNetwork::contractAll(order) / contractBond (reference
+ * src/tensor/network.cpp:60-170), whose steps place their results as
+ * Tensor::contract does (contract.cpp:219-289). */
+qtnh_status qtnh_net_plan_steps(qtnh_net_t net, int nsteps, const qtnh_net_step_spec *steps);
 /* Arithmetic of the network's large (GEMM-route) steps: QTNH_PREC_FP64 (default)
  * or QTNH_PREC_SPLIT (the split-precision GEMM of qtnh_contract_c128, for steps
  * whose result fills the GPU with at least 256 tiles of 128 x 128 over the
@@ -243,6 +273,11 @@ qtnh_status qtnh_net_step(qtnh_net_t net, uint64_t s, int *a, int *b, int *c, in
 /* Contracts the planned network, one launch per step; out (host) receives the
  * result (out_cap complex elements at least).  Blocks. */
 qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t out_cap);
+/* Contracts the planned network on the context's stream into device buffer
+ * out (the result's element count), without blocking: intermediates live in
+ * the context's workspace, the last step writes out directly.  out must not
+ * alias a leaf. */
+qtnh_status qtnh_net_run_device(qtnh_ctx_t ctx, qtnh_net_t net, void *out);
 /* Contracts planned networks together: step k of every network in one grouped
  * launch (small binary contractions) plus the large steps in turn.  Networks
  * are admitted in consecutive waves whose device blocks fit budget_bytes.
@@ -273,6 +308,14 @@ qtnh_status qtnh_net_batch_destroy(qtnh_net_batch_t b);
 #define QTNH_SERVE_BATCHED 1    /* continuous batching: steps of grouped fused runs over every circuit
                                    in flight; arrivals admitted FIFO at each step while their states
                                    fit the budget; least-recently-used resident output states */
+#define QTNH_SERVE_SEQUENTIAL_FUSED 2 /* one query at a time, fused runs, no resident states */
+#define QTNH_SERVE_NETWORK 3    /* as BATCHED, but each amplitude query takes the cheaper route by
+                                   estimated time: its state vector, or its tensor network <x|C|0>
+                                   (best of four greedy orders within rank 28), contracted
+                                   with the step's other network queries in one grouped batch */
+/* Flags for QTNH_SERVE_NETWORK (or-ed into the policy): */
+#define QTNH_SERVE_SPLIT 0x100       /* network GEMM steps that fill the GPU in split precision */
+#define QTNH_SERVE_ALL_NETWORK 0x200 /* every amplitude query whose plan fits takes the network route */
 typedef struct {
   int n_qubits;
   int depth;
@@ -287,9 +330,11 @@ typedef struct {
   double zsum;
   int batch;  /* batched: step at which it was admitted; sequential: its position in service order */
   int cached; /* 1: answered from a resident state */
+  int route;  /* 0: state vector, 1: tensor network */
 } qtnh_query_result;
-/* Serves the queries in arrival order; returns when all are answered.
- * Answers are identical under both policies. */
+/* Serves the queries in arrival order; returns when all are answered.  A
+ * query's answer does not depend on the queries served with it; state-vector
+ * answers are identical under every policy. */
 qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *queries, qtnh_query_result *results,
                        uint64_t budget_bytes, int policy);
 

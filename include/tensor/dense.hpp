@@ -79,6 +79,7 @@ namespace qtnh {
     public:
       friend class DenseTensorBase;
       friend class Tensor;
+      friend class TensorNetwork;
 
       DenseTensor() = delete;
       DenseTensor(const DenseTensor&) = delete;

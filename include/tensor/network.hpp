@@ -1,7 +1,8 @@
 // Tensor networks (reference include/tensor/network.hpp:1-111): the caller of
 // the contraction path.  Bonds hold the wires between two tensors; contracting
 // a bond re-keys the remaining bonds' wires through the contraction's index
-// placement (reference network.cpp:60-106).
+// placement (reference network.cpp:60-106).  Contractions run as one planned
+// device network per call (no host round trip between steps).
 #ifndef QTNH_TENSOR_NETWORK_HPP
 #define QTNH_TENSOR_NETWORK_HPP
 
@@ -53,7 +54,16 @@ namespace qtnh {
 
       void print();
 
+      /// Extension: steps of the device plan the last contractBond / contractAll
+      /// ran (identity tensors contracted in place add none).
+      std::size_t lastPlanSteps() const noexcept;
+
     private:
+      /// Plans bids in order on metadata (merging consecutive bonds between the
+      /// same tensors when `merge`), then runs the steps on the device.
+      qtnh::uint run(const std::vector<qtnh::uint>& bids, bool merge, bool rebcast);
+      std::size_t last_plan_steps_ = 0;
+
       // Shared by all networks, as in the reference.
       inline static qtnh::uint tensor_counter = 0;
       inline static qtnh::uint bond_counter = 0;

@@ -15,7 +15,8 @@ PREC_FP64, PREC_SPLIT = 0, 1
 NOALLOC_SCRATCH_BYTES = 64 << 20  # QTNH_NOALLOC_SCRATCH_BYTES
 RUN_PER_GATE, RUN_FUSED = 0, 1
 QUERY_AMPLITUDE, QUERY_EXPECT_Z = 0, 1
-SERVE_SEQUENTIAL, SERVE_BATCHED = 0, 1
+SERVE_SEQUENTIAL, SERVE_BATCHED, SERVE_SEQUENTIAL_FUSED, SERVE_NETWORK = 0, 1, 2, 3
+SERVE_SPLIT, SERVE_ALL_NETWORK = 0x100, 0x200  # flags of SERVE_NETWORK
 
 
 class Query(ctypes.Structure):
@@ -25,7 +26,7 @@ class Query(ctypes.Structure):
 
 class QueryResult(ctypes.Structure):
     _fields_ = [("start_s", ctypes.c_double), ("done_s", ctypes.c_double), ("value", ctypes.c_double * 2),
-                ("zsum", ctypes.c_double), ("batch", ctypes.c_int), ("cached", ctypes.c_int)]
+                ("zsum", ctypes.c_double), ("batch", ctypes.c_int), ("cached", ctypes.c_int), ("route", ctypes.c_int)]
 DIST_NORMAL, DIST_UNIFORM = 0, 1
 
 _p = ctypes.c_void_p
@@ -83,6 +84,10 @@ SIGNATURES = {
     "qtnh_net_create": [ctypes.POINTER(_p)],
     "qtnh_net_destroy": [_p],
     "qtnh_net_add_tensor": [_p, _i, _u64p, ctypes.POINTER(ctypes.c_int64), _dp, ctypes.POINTER(_i)],
+    "qtnh_net_add_tensor_device": [_p, _i, _u64p, ctypes.POINTER(ctypes.c_int64), _p, ctypes.POINTER(_i)],
+    "qtnh_net_plan_steps": [_p, _i, _p],
+    "qtnh_net_plan_best": [_p, _u64, _i, _i, _u64p],
+    "qtnh_net_run_device": [_p, _p, _p],
     "qtnh_net_from_circuit": [_p, _p, ctypes.POINTER(ctypes.c_uint8)],
     "qtnh_net_plan": [_p, _u64, _i],
     "qtnh_net_set_precision": [_p, _i],

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <stdexcept>
 
+#include "placement.hpp"
 #include "status.hpp"
 #include "tensor/dense.hpp"
 #include "tensor/diag.hpp"
@@ -62,68 +63,80 @@ namespace qtnh {
     }
   }
 
-  qtnh::tptr Tensor::contract(qtnh::tptr tp1, qtnh::tptr tp2, ConParams& params) {
-    const QTNHEnv& env = tp1->bc().env;
-    const auto d1 = tp1->totDims(), d2 = tp2->totDims();
-    const int r1 = (int)d1.size(), r2 = (int)d2.size();
-    const int nd1 = (int)tp1->disDims().size(), nd2 = (int)tp2->disDims().size();
-    int n_dis_ws = 0;
-    for (auto& w : params.wires) {
-      if (d1.at(w.first) != d2.at(w.second)) throw std::invalid_argument("Incompatible contraction dimensions.");
-      if ((w.first < nd1) != (w.second < nd2))
-        throw std::invalid_argument("Attempted to contract distributed and local dimensions.");
-      if (w.first < nd1) ++n_dis_ws;
-    }
-    const int nw = (int)params.wires.size();
-    auto byfirst = [](const qtnh::wire& a, const qtnh::wire& b) { return a.first < b.first; };
-    auto bysecond = [](const qtnh::wire& a, const qtnh::wire& b) { return a.second < b.second; };
+  namespace detail {
+    Placement place(const qtnh::tidx_tup& d1, int nd1, const qtnh::tidx_tup& d2, int nd2, int symm_n_dis_in,
+                    ConParams& params) {
+      const int r1 = (int)d1.size(), r2 = (int)d2.size();
+      int n_dis_ws = 0;
+      for (auto& w : params.wires) {
+        if (d1.at(w.first) != d2.at(w.second)) throw std::invalid_argument("Incompatible contraction dimensions.");
+        if (((int)w.first < nd1) != ((int)w.second < nd2))
+          throw std::invalid_argument("Attempted to contract distributed and local dimensions.");
+        if ((int)w.first < nd1) ++n_dis_ws;
+      }
+      const int nw = (int)params.wires.size();
+      auto byfirst = [](const qtnh::wire& a, const qtnh::wire& b) { return a.first < b.first; };
+      auto bysecond = [](const qtnh::wire& a, const qtnh::wire& b) { return a.second < b.second; };
 
-    if (params.useDefRepls) {
-      std::vector<uint16_t> o1(std::max(r1, 1)), o2(std::max(r2, 1));
-      auto fw = flat_wires(params.wires);
-      int st = qtnh_default_repls(r1, nd1, r2, nd2, nw, fw.data(), o1.data(), o2.data());
-      if (st != QTNH_OK) throw std::invalid_argument("Invalid contraction wires.");
-      params.dimRepls1.assign(o1.begin(), o1.begin() + r1);
-      params.dimRepls2.assign(o2.begin(), o2.begin() + r2);
-      std::sort(params.wires.begin(), params.wires.end(), byfirst);
-      std::sort(params.wires.begin(), params.wires.end(), bysecond);
-    }
-
-    SymmTensorBase* s2 = dynamic_cast<SymmTensorBase*>(tp2.get());
-    bool symm = false;
-    if (s2) {
-      std::vector<uint16_t> o1(std::max(r1, 1)), o2(std::max(r2, 1));
-      auto fw = flat_wires(params.wires);
-      int applies = 0;
-      int st = qtnh_symm_repls(r1, nd2, r2 - nd2, s2->nDisIn(), nw, fw.data(), o1.data(), o2.data(), &applies);
-      if (st != QTNH_OK) throw std::invalid_argument("Invalid contraction wires.");
-      if (applies) {
+      if (params.useDefRepls) {
+        std::vector<uint16_t> o1(std::max(r1, 1)), o2(std::max(r2, 1));
+        auto fw = flat_wires(params.wires);
+        int st = qtnh_default_repls(r1, nd1, r2, nd2, nw, fw.data(), o1.data(), o2.data());
+        if (st != QTNH_OK) throw std::invalid_argument("Invalid contraction wires.");
         params.dimRepls1.assign(o1.begin(), o1.begin() + r1);
         params.dimRepls2.assign(o2.begin(), o2.begin() + r2);
         std::sort(params.wires.begin(), params.wires.end(), byfirst);
         std::sort(params.wires.begin(), params.wires.end(), bysecond);
-        symm = true;
       }
-    }
-    if ((int)params.dimRepls1.size() != r1 || (int)params.dimRepls2.size() != r2)
-      throw std::out_of_range("Index replacements do not match the tensors.");
 
-    // Result dims under the placement; a placement that is not a permutation
-    // of the result indices is rejected (the reference indexes past its vectors).
-    auto fw = flat_wires(params.wires);
-    auto rp1 = u16(params.dimRepls1), rp2 = u16(params.dimRepls2);
-    int rc = 0;
-    std::vector<uint64_t> dC(QTNH_MAX_RANK);
-    if (qtnh_contract_dims(r1, d1.data(), r2, d2.data(), nw, fw.data(), rp1.data(), rp2.data(), &rc, dC.data()) !=
-        QTNH_OK)
-      throw std::out_of_range("Invalid index replacements.");
-    dC.resize(rc);
-    const int ndC = (nd1 - n_dis_ws) + (nd2 - n_dis_ws);
-    qtnh::tidx_tup dCt(dC.begin(), dC.end());
-    auto split = utils::split_dims(dCt, (qtnh::tidx_tup_st)ndC);
+      Placement P;
+      if (symm_n_dis_in >= 0) {
+        std::vector<uint16_t> o1(std::max(r1, 1)), o2(std::max(r2, 1));
+        auto fw = flat_wires(params.wires);
+        int applies = 0;
+        int st = qtnh_symm_repls(r1, nd2, r2 - nd2, symm_n_dis_in, nw, fw.data(), o1.data(), o2.data(), &applies);
+        if (st != QTNH_OK) throw std::invalid_argument("Invalid contraction wires.");
+        if (applies) {
+          params.dimRepls1.assign(o1.begin(), o1.begin() + r1);
+          params.dimRepls2.assign(o2.begin(), o2.begin() + r2);
+          std::sort(params.wires.begin(), params.wires.end(), byfirst);
+          std::sort(params.wires.begin(), params.wires.end(), bysecond);
+          P.symm = true;
+        }
+      }
+      if ((int)params.dimRepls1.size() != r1 || (int)params.dimRepls2.size() != r2)
+        throw std::out_of_range("Index replacements do not match the tensors.");
+
+      // Result dims under the placement; a placement that is not a permutation
+      // of the result indices is rejected (the reference indexes past its vectors).
+      P.wires = flat_wires(params.wires);
+      P.repl1 = u16(params.dimRepls1);
+      P.repl2 = u16(params.dimRepls2);
+      int rc = 0;
+      std::vector<uint64_t> dC(QTNH_MAX_RANK);
+      if (qtnh_contract_dims(r1, d1.data(), r2, d2.data(), nw, P.wires.data(), P.repl1.data(), P.repl2.data(), &rc,
+                             dC.data()) != QTNH_OK)
+        throw std::out_of_range("Invalid index replacements.");
+      P.dims.assign(dC.begin(), dC.begin() + rc);
+      P.ndis = (nd1 - n_dis_ws) + (nd2 - n_dis_ws);
+      P.in_place = P.symm && nw * 2 == r2 && P.dims == d1;
+      return P;
+    }
+  }
+
+  qtnh::tptr Tensor::contract(qtnh::tptr tp1, qtnh::tptr tp2, ConParams& params) {
+    const QTNHEnv& env = tp1->bc().env;
+    const auto d1 = tp1->totDims(), d2 = tp2->totDims();
+    const int r1 = (int)d1.size(), r2 = (int)d2.size();
+    SymmTensorBase* s2 = dynamic_cast<SymmTensorBase*>(tp2.get());
+    const detail::Placement P = detail::place(d1, (int)tp1->disDims().size(), d2, (int)tp2->disDims().size(),
+                                              s2 ? (int)s2->nDisIn() : -1, params);
+    const int nw = (int)params.wires.size();
+    const qtnh::tidx_tup& dCt = P.dims;
+    auto split = utils::split_dims(dCt, (qtnh::tidx_tup_st)P.ndis);
 
     // In-place route: symmetric placement, exactly the inputs wired, result dims = state dims.
-    if (symm && nw * 2 == r2 && dCt == d1) {
+    if (P.in_place) {
       std::vector<std::size_t> ins, outs, axes;
       std::vector<char> wired(r2, 0);
       for (auto& w : params.wires) {  // sorted by second
@@ -175,8 +188,8 @@ namespace qtnh {
       void* pc = out->device_out();
       double err = 0.0;
       detail::check(c,
-                    qtnh_contract_c128(c, pa, r1, d1.data(), pb, r2, d2.data(), nw, fw.data(), rp1.data(), rp2.data(),
-                                       pc, env.precision, &err),
+                    qtnh_contract_c128(c, pa, r1, d1.data(), pb, r2, d2.data(), nw, P.wires.data(), P.repl1.data(),
+                                       P.repl2.data(), pc, env.precision, &err),
                     "contraction");
       env.last_split_error = err;
     }

@@ -153,11 +153,20 @@ namespace qtnh {
   qtnh::tel Tensor::fetch(qtnh::tidx_tup tot_idxs) const { return at(tot_idxs); }
 
   namespace ops {
+    // Format of reference src/tensor/tensor.cpp:86-105: elements in row-major
+    // order at two significant digits, ", " between elements except after the
+    // last element of each local slice.  One GPU holds every slice, so every
+    // element is printed (slices follow each other with no separator).
     std::ostream& operator<<(std::ostream& out, const Tensor& o) {
+      if (!o.bc().active) {
+        out << "Inactive";
+        return out;
+      }
       out << std::setprecision(2);
       TIndexing ti(o.totDims());
       const std::size_t ls = std::max<std::size_t>(o.locSize(), 1);
       for (auto idxs : ti.tup()) {
+        if (!o.has(idxs)) continue;
         out << o.at(idxs);
         if ((utils::idxs_to_i(idxs, o.totDims()) + 1) % ls != 0) out << ", ";
       }

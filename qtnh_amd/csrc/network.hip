@@ -140,10 +140,16 @@ struct qtnh_net_s {
     std::vector<int64_t> bonds;
     uint64_t size = 1;
     uint64_t offset = 0;  // leaves: in the leaf block; intermediates: in the arena
+    const void *ext = nullptr;  // leaves read from caller device memory (no host copy)
   };
   struct Step {
     int a, b, c;                  // node ids (a is the contraction's first tensor)
     std::vector<uint16_t> wires;  // (axis of a, axis of b) pairs
+    // placed: repl1 / repl2 hold the result position of each axis of a and
+    // of b (QTNH_X for wired axes); else the default placement (open axes of
+    // a, then of b).
+    bool placed = false;
+    std::vector<uint16_t> repl1, repl2;
     bool direct = true;
     bool binary = true;
     uint64_t ws = 0;
@@ -184,11 +190,26 @@ int step_precision(const qtnh_net_s::Step &st, int precision) {
 }
 
 // Route launch_contract takes for a step (default placement) at a precision.
-ContractRoute step_route(const qtnh_net_s &net, const qtnh_net_s::Step &st, int precision) {
+// Result placement of a step: its own, or the default one.
+void step_repls(const qtnh_net_s &net, const qtnh_net_s::Step &st, std::vector<uint16_t> &r1,
+                std::vector<uint16_t> &r2) {
   const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
-  std::vector<uint16_t> r1(std::max<size_t>(A.dims.size(), 1)), r2(std::max<size_t>(B.dims.size(), 1));
+  r1.assign(std::max<size_t>(A.dims.size(), 1), 0);
+  r2.assign(std::max<size_t>(B.dims.size(), 1), 0);
+  if (st.placed) {
+    std::copy(st.repl1.begin(), st.repl1.end(), r1.begin());
+    std::copy(st.repl2.begin(), st.repl2.end(), r2.begin());
+    return;
+  }
   const int nw = (int)st.wires.size() / 2;
   default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
+}
+
+ContractRoute step_route(const qtnh_net_s &net, const qtnh_net_s::Step &st, int precision) {
+  const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
+  std::vector<uint16_t> r1, r2;
+  step_repls(net, st, r1, r2);
+  const int nw = (int)st.wires.size() / 2;
   return contract_route((int)A.dims.size(), A.dims.data(), (int)B.dims.size(), B.dims.data(), nw, st.wires.data(),
                         r1.data(), r2.data(), precision);
 }
@@ -234,20 +255,72 @@ struct Arena {
   }
 };
 
-qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
+// Appends step st with result node r: its flop and byte counts, the route and
+// workspace launch_contract will take, and r's arena offset (taken before the
+// operands are released, so a result never aliases an operand); intermediate
+// operands go back to the arena.
+void add_step(qtnh_net_s &net, qtnh_net_s::Step st, qtnh_net_s::Node r, Arena &arena) {
+  const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
+  uint64_t K = 1;
+  for (size_t w = 0; w < st.wires.size(); w += 2) K *= A.dims[st.wires[w]];
+  st.flops = 8.0 * (double)r.size * (double)K;
+  st.bytes = 16.0 * ((double)A.size + (double)B.size + (double)r.size);
+  st.binary = true;
+  for (uint64_t d : A.dims) st.binary = st.binary && d == 2;
+  for (uint64_t d : B.dims) st.binary = st.binary && d == 2;
+  st.binary = st.binary && A.dims.size() <= kGMaxBits && B.dims.size() <= kGMaxBits && r.dims.size() <= kGMaxBits;
+  const ContractRoute R = step_route(net, st, QTNH_PREC_FP64);
+  st.direct = R.direct;
+  st.M = R.M;
+  st.N = R.N;
+  st.ws = step_precision(st, net.precision) == QTNH_PREC_SPLIT ? step_route(net, st, QTNH_PREC_SPLIT).ws_bytes
+                                                                : R.ws_bytes;
+  net.ws_bytes = std::max(net.ws_bytes, st.ws);
+  r.offset = arena.alloc(r.size);
+  net.max_rank = std::max(net.max_rank, (int)r.dims.size());
+  net.flops += st.flops;
+  net.bytes += st.bytes;
+  st.c = (int)net.nodes.size();
+  for (int x : {st.a, st.b})
+    if ((size_t)x >= net.n_leaves) arena.release(net.nodes[x].offset, net.nodes[x].size);
+  net.nodes.push_back(std::move(r));
+  net.steps.push_back(std::move(st));
+}
+
+// Arena size, result node and grouped-launch templates of a finished order.
+void finish_plan(qtnh_net_s &net) {
+  uint64_t hw = 0;
+  for (const auto &st : net.steps) {
+    const auto &r = net.nodes[st.c];
+    hw = std::max(hw, r.offset + round16(std::max<uint64_t>(r.size, 1)));
+  }
+  net.arena_elems = hw;
+  net.result = net.steps.empty() ? 0 : net.steps.back().c;
+  net.gdesc.assign(net.steps.size(), GDesc{});
+  for (size_t s = 0; s < net.steps.size(); ++s)
+    if (net.steps[s].direct && net.steps[s].binary) net.gdesc[s] = gdesc_template(net, s);
+  net.planned = true;
+}
+
+void reset_plan(qtnh_net_s &net) {
   net.steps.clear();
   net.nodes.resize(net.n_leaves);
   net.planned = false;
   net.max_rank = 0;
   net.flops = net.bytes = 0;
+  net.ws_bytes = 0;
+  net.arena_elems = 0;
+  for (const auto &x : net.nodes) net.max_rank = std::max(net.max_rank, (int)x.dims.size());
+}
+
+qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
+  reset_plan(net);
   const size_t L = net.n_leaves;
   if (L == 0) return QTNH_EINVAL;
   std::vector<char> live(L, 1);
   std::map<int64_t, std::vector<int>> holders;  // bond -> live nodes carrying it
-  for (size_t i = 0; i < L; ++i) {
-    net.max_rank = std::max(net.max_rank, (int)net.nodes[i].dims.size());
+  for (size_t i = 0; i < L; ++i)
     for (int64_t b : net.nodes[i].bonds) holders[b].push_back((int)i);
-  }
   for (auto &h : holders)
     if (h.second.size() > 2) return QTNH_EINVAL;  // a bond joins at most two axes
   // Result of contracting a and b: open axes of a, then open axes of b.
@@ -323,38 +396,13 @@ qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
     if ((int)r.dims.size() > max_rank || (int)r.dims.size() > QTNH_MAX_RANK) return QTNH_EUNSUPPORTED;
     st.a = a;
     st.b = b;
-    st.c = (int)net.nodes.size();
-    const auto &A = net.nodes[a], &B = net.nodes[b];
-    uint64_t K = 1;
-    for (size_t w = 0; w < st.wires.size(); w += 2) K *= A.dims[st.wires[w]];
-    st.flops = 8.0 * (double)r.size * (double)K;
-    st.bytes = 16.0 * ((double)A.size + (double)B.size + (double)r.size);
-    for (uint64_t d : A.dims) st.binary = st.binary && d == 2;
-    for (uint64_t d : B.dims) st.binary = st.binary && d == 2;
-    st.binary = st.binary && A.dims.size() <= kGMaxBits && B.dims.size() <= kGMaxBits && r.dims.size() <= kGMaxBits;
-    // Route and workspace, as launch_contract will choose them.
-    const ContractRoute R = step_route(net, st, QTNH_PREC_FP64);
-    st.direct = R.direct;
-    st.M = R.M;
-    st.N = R.N;
-    st.ws = step_precision(st, net.precision) == QTNH_PREC_SPLIT ? step_route(net, st, QTNH_PREC_SPLIT).ws_bytes
-                                                                  : R.ws_bytes;
-    net.ws_bytes = std::max(net.ws_bytes, st.ws);
-    // Arena: the result is placed before its inputs are released (no aliasing).
-    r.offset = arena.alloc(r.size);
-    net.max_rank = std::max(net.max_rank, (int)r.dims.size());
-    net.flops += st.flops;
-    net.bytes += st.bytes;
-    net.nodes.push_back(r);
+    add_step(net, std::move(st), std::move(r), arena);
     live.push_back(1);
-    for (int x : {a, b}) {
-      live[x] = 0;
-      if ((size_t)x >= L) arena.release(net.nodes[x].offset, net.nodes[x].size);
-    }
+    live[a] = 0;
+    live[b] = 0;
     --live_count;
-    net.steps.push_back(st);
     // New candidate pairs through the result's bonds.
-    const int c = st.c;
+    const int c = net.steps.back().c;
     for (int64_t bond : net.nodes[c].bonds) {
       auto &h = holders[bond];
       for (int &x : h)
@@ -362,29 +410,71 @@ qtnh_status plan_greedy(qtnh_net_s &net, uint64_t seed, int max_rank) {
       if (h.size() == 2 && live[h[0]] && live[h[1]]) push_pair(h[0], h[1]);
     }
   }
-  // High-water mark of the arena.
-  uint64_t hw = 0;
-  for (const auto &st : net.steps) {
-    const auto &r = net.nodes[st.c];
-    hw = std::max(hw, r.offset + round16(std::max<uint64_t>(r.size, 1)));
-  }
-  net.arena_elems = hw;
-  net.result = net.steps.empty() ? 0 : net.steps.back().c;
-  net.gdesc.assign(net.steps.size(), GDesc{});
-  for (size_t s = 0; s < net.steps.size(); ++s)
-    if (net.steps[s].direct && net.steps[s].binary) net.gdesc[s] = gdesc_template(net, s);
-  net.planned = true;
+  finish_plan(net);
   return QTNH_OK;
 }
 
-// Device placement of one network: its leaves and its arena.
+// A caller-given order: step s contracts nodes specs[s].a and .b (leaves
+// 0..L-1, the result of step s is node L+s) over the given wires, placed as
+// given (default placement without repl arrays).  Every node but the last
+// result must be consumed exactly once.
+qtnh_status plan_steps(qtnh_net_s &net, int nsteps, const qtnh_net_step_spec *specs) {
+  reset_plan(net);
+  const size_t L = net.n_leaves;
+  if (L == 0 || nsteps < 0 || (nsteps > 0 && !specs)) return QTNH_EINVAL;
+  if ((size_t)nsteps != L - 1) return QTNH_EINVAL;  // each step joins two tensors into one
+  std::vector<char> live(L, 1);
+  Arena arena;
+  for (int s = 0; s < nsteps; ++s) {
+    const qtnh_net_step_spec &sp = specs[s];
+    const int n = (int)net.nodes.size();
+    if (sp.a < 0 || sp.b < 0 || sp.a >= n || sp.b >= n || sp.a == sp.b || !live[sp.a] || !live[sp.b] ||
+        sp.nw < 0 || (sp.nw > 0 && !sp.wires) || (!sp.repl1) != (!sp.repl2))
+      return QTNH_EINVAL;
+    const auto &A = net.nodes[sp.a], &B = net.nodes[sp.b];
+    const int rA = (int)A.dims.size(), rB = (int)B.dims.size();
+    qtnh_net_s::Step st;
+    st.a = sp.a;
+    st.b = sp.b;
+    st.wires.assign(sp.wires, sp.wires + 2 * sp.nw);
+    if (sp.repl1) {
+      st.placed = true;
+      st.repl1.assign(sp.repl1, sp.repl1 + rA);
+      st.repl2.assign(sp.repl2, sp.repl2 + rB);
+    }
+    std::vector<uint16_t> r1, r2;
+    step_repls(net, st, r1, r2);
+    int rC = 0;
+    uint64_t dC[QTNH_MAX_RANK];
+    if (contract_dims(rA, A.dims.data(), rB, B.dims.data(), sp.nw, st.wires.data(), r1.data(), r2.data(), &rC, dC) !=
+        QTNH_OK)
+      return QTNH_EINVAL;
+    qtnh_net_s::Node r;
+    r.dims.assign(dC, dC + rC);
+    r.size = 1;
+    for (uint64_t d : r.dims) r.size *= d;
+    r.bonds.assign(rC, -1);  // labels are not used by a given order
+    live[sp.a] = 0;
+    live[sp.b] = 0;
+    live.push_back(1);
+    add_step(net, std::move(st), std::move(r), arena);
+  }
+  finish_plan(net);
+  return QTNH_OK;
+}
+
+// Device placement of one network: its leaves and its arena (and, when set,
+// the buffer its result node is written to instead of the arena).
 struct Where {
   c128 *leaves;
   c128 *arena;
+  c128 *result = nullptr;
 };
 
 c128 *node_ptr(const qtnh_net_s &net, Where block, int x) {
-  if ((size_t)x < net.n_leaves) return block.leaves + net.nodes[x].offset;
+  if ((size_t)x < net.n_leaves)
+    return net.nodes[x].ext ? (c128 *)net.nodes[x].ext : block.leaves + net.nodes[x].offset;
+  if (block.result && x == net.result) return block.result;
   return block.arena + net.nodes[x].offset;
 }
 
@@ -392,9 +482,9 @@ qtnh_status run_step(qtnh_ctx_t ctx, const qtnh_net_s &net, Where block, size_t 
                      int precision) {
   const auto &st = net.steps[s];
   const auto &A = net.nodes[st.a], &B = net.nodes[st.b];
-  std::vector<uint16_t> r1(std::max<size_t>(A.dims.size(), 1)), r2(std::max<size_t>(B.dims.size(), 1));
+  std::vector<uint16_t> r1, r2;
+  step_repls(net, st, r1, r2);
   const int nw = (int)st.wires.size() / 2;
-  default_repls((int)A.dims.size(), 0, (int)B.dims.size(), 0, nw, st.wires.data(), r1.data(), r2.data());
   return launch_contract(ctx, node_ptr(net, block, st.a), (int)A.dims.size(), A.dims.data(), node_ptr(net, block, st.b),
                          (int)B.dims.size(), B.dims.data(), nw, st.wires.data(), r1.data(), r2.data(),
                          node_ptr(net, block, st.c), step_precision(st, precision), nullptr, ws, ws_bytes);
@@ -414,7 +504,10 @@ GDesc gdesc_template(const qtnh_net_s &net, size_t s) {
   d.ncb = rC;
   const int nw = (int)st.wires.size() / 2;
   d.kbits = nw;
-  // Result axis q <- open axes of A in order, then of B; bit p = rC - 1 - q.
+  // Open axis i of A goes to result axis r1[i] (of B: r2[i]); result axis q is
+  // bit rC - 1 - q.
+  std::vector<uint16_t> r1, r2;
+  step_repls(net, st, r1, r2);
   std::vector<char> wa(rA, 0), wb(rB, 0);
   std::vector<std::pair<int, int>> w(nw);
   for (int i = 0; i < nw; ++i) {
@@ -422,17 +515,12 @@ GDesc gdesc_template(const qtnh_net_s &net, size_t s) {
     wa[w[i].first] = 1;
     wb[w[i].second] = 1;
   }
-  int q = 0;
   for (int i = 0; i < rA; ++i)
-    if (!wa[i]) {
-      d.abit[rC - 1 - q] = (int8_t)(rA - 1 - i);
-      ++q;
-    }
+    if (!wa[i]) d.abit[rC - 1 - r1[i]] = (int8_t)(rA - 1 - i);
   for (int i = 0; i < rB; ++i)
     if (!wb[i]) {
-      d.abit[rC - 1 - q] = -1;
-      d.bbit[rC - 1 - q] = (int8_t)(rB - 1 - i);
-      ++q;
+      d.abit[rC - 1 - r2[i]] = -1;
+      d.bbit[rC - 1 - r2[i]] = (int8_t)(rB - 1 - i);
     }
   // Sum order: wires sorted by their axis in B, the last one fastest.
   std::sort(w.begin(), w.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
@@ -689,6 +777,31 @@ qtnh_status qtnh_net_add_tensor(qtnh_net_t net, int rank, const uint64_t *dims, 
   return QTNH_OK;
 }
 
+qtnh_status qtnh_net_add_tensor_device(qtnh_net_t net, int rank, const uint64_t *dims, const int64_t *bonds,
+                                       const void *data, int *id) {
+  if (!net || rank < 0 || rank > QTNH_MAX_RANK || (rank > 0 && !dims) || !data) return QTNH_EINVAL;
+  if (net->nodes.size() != net->n_leaves) return QTNH_EINVAL;  // leaves only before planning
+  qtnh_net_s::Node x;
+  uint64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] == 0 || n > UINT64_MAX / dims[i]) return QTNH_EINVAL;
+    if (bonds)
+      for (int j = 0; j < i; ++j)
+        if (bonds[j] == bonds[i]) return QTNH_EINVAL;
+    n *= dims[i];
+    x.dims.push_back(dims[i]);
+    // Without labels each axis gets one of its own (never shared).
+    x.bonds.push_back(bonds ? bonds[i] : INT64_MIN + (int64_t)(net->nodes.size() * QTNH_MAX_RANK + i));
+  }
+  x.size = n;
+  x.ext = data;
+  if (id) *id = (int)net->nodes.size();
+  net->nodes.push_back(x);
+  net->n_leaves = net->nodes.size();
+  net->planned = false;
+  return QTNH_OK;
+}
+
 qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_t *bits) {
   if (!net || !c) return QTNH_EINVAL;
   if (!net->nodes.empty()) return QTNH_EINVAL;
@@ -744,6 +857,36 @@ qtnh_status qtnh_net_from_circuit(qtnh_net_t net, qtnh_circuit_t c, const uint8_
 qtnh_status qtnh_net_plan(qtnh_net_t net, uint64_t seed, int max_rank) {
   if (!net || max_rank < 0) return QTNH_EINVAL;
   return plan_greedy(*net, seed, max_rank);
+}
+
+qtnh_status qtnh_net_plan_best(qtnh_net_t net, uint64_t seed, int max_rank, int tries, uint64_t *chosen_seed) {
+  if (!net || max_rank < 0 || tries < 1) return QTNH_EINVAL;
+  // Greedy plans are deterministic per seed: time nothing, keep the seed of
+  // the fewest flops, plan it again.
+  double best = 0;
+  uint64_t best_seed = 0;
+  bool found = false;
+  for (int t = 0; t < tries; ++t) {
+    const qtnh_status st = plan_greedy(*net, seed + (uint64_t)t, max_rank);
+    if (st == QTNH_EUNSUPPORTED) continue;
+    if (st) return st;
+    if (!found || net->flops < best) {
+      best = net->flops;
+      best_seed = seed + (uint64_t)t;
+      found = true;
+    }
+  }
+  if (!found) {
+    reset_plan(*net);
+    return QTNH_EUNSUPPORTED;
+  }
+  if (chosen_seed) *chosen_seed = best_seed;
+  return plan_greedy(*net, best_seed, max_rank);
+}
+
+qtnh_status qtnh_net_plan_steps(qtnh_net_t net, int nsteps, const qtnh_net_step_spec *steps) {
+  if (!net) return QTNH_EINVAL;
+  return plan_steps(*net, nsteps, steps);
 }
 
 qtnh_status qtnh_net_set_precision(qtnh_net_t net, int precision) {
@@ -818,6 +961,38 @@ qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t o
                  "net_run result copy");
   if (st) return st;
   return check_hip(ctx, hipStreamSynchronize(ctx->stream), "net_run sync");
+}
+
+qtnh_status qtnh_net_run_device(qtnh_ctx_t ctx, qtnh_net_t net, void *out) {
+  qtnh_status st = enter_ctx(ctx);
+  if (st) return st;
+  if (!net || !out) return set_error(ctx, QTNH_EINVAL, "net_run_device: null argument");
+  if (!net->planned) return set_error(ctx, QTNH_EINVAL, "net_run_device: network has no plan");
+  for (size_t i = 0; i < net->n_leaves; ++i)
+    if (net->nodes[i].ext == out) return set_error(ctx, QTNH_EINVAL, "net_run_device: out aliases leaf %zu", i);
+  const uint64_t rsize = net->nodes[net->result].size;
+  const uint64_t elems = block_elems(*net);
+  void *w = nullptr;
+  st = workspace(ctx, elems * sizeof(c128) + net->ws_bytes, &w);
+  if (st) return st;
+  Where block = {(c128 *)w, (c128 *)w + round16(net->leaf_elems), (c128 *)out};
+  char *ws = (char *)((c128 *)w + elems);
+  if (net->leaf_elems)
+    st = check_hip(ctx, hipMemcpyAsync(block.leaves, net->leaf_data.data(), net->leaf_elems * sizeof(c128),
+                                       hipMemcpyHostToDevice, ctx->stream),
+                   "net_run_device leaf upload");
+  if (st) return st;
+  if (net->steps.empty()) {
+    // A single leaf: the result is its copy.
+    const c128 *src = net->nodes[0].ext ? (const c128 *)net->nodes[0].ext : block.leaves + net->nodes[0].offset;
+    return check_hip(ctx, hipMemcpyAsync(out, src, rsize * sizeof(c128), hipMemcpyDeviceToDevice, ctx->stream),
+                     "net_run_device copy");
+  }
+  for (size_t s = 0; s < net->steps.size(); ++s) {
+    st = run_step(ctx, *net, block, s, ws, net->ws_bytes, net->precision);
+    if (st) return st;
+  }
+  return QTNH_OK;
 }
 
 qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets, double *const *outs,
@@ -934,6 +1109,8 @@ qtnh_status qtnh_net_batch_set_leaf(qtnh_ctx_t ctx, qtnh_net_batch_t b, int net,
     return set_error(ctx, QTNH_EINVAL, "net_batch_set_leaf: bad arguments");
   const qtnh_net_s &n = *b->nets[net];
   if (leaf < 0 || (size_t)leaf >= n.n_leaves) return set_error(ctx, QTNH_EINVAL, "net_batch_set_leaf: no leaf %d", leaf);
+  if (n.nodes[leaf].ext)
+    return set_error(ctx, QTNH_EINVAL, "net_batch_set_leaf: leaf %d is read from caller device memory", leaf);
   c128 *dst = node_ptr(n, b->wave.blocks[net], leaf);
   return check_hip(ctx, hipMemcpyAsync(dst, data, n.nodes[leaf].size * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
                    "net_batch_set_leaf copy");

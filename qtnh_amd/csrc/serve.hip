@@ -6,9 +6,11 @@
 // times, are admitted first-in first-out while the states they need fit the
 // budget, and are answered from resident states when the same circuit's
 // output is still in HBM (least recently used states are evicted to make
-// room).  Two policies:
+// room).  Policies:
 //  * sequential: one query at a time, gates applied one launch each (the
 //    reference's one contraction per gate), no resident states;
+//  * sequential fused: one query at a time, the circuit applied as fused runs
+//    (the same executor as the batched policy, without batching);
 //  * batched (continuous batching): the loop runs in steps.  Before each step
 //    it admits arrived queries while their states fit (a query whose
 //    circuit is already in flight joins it; one whose output state is
@@ -17,9 +19,18 @@
 //    by up to 8 runs, larger states by one), the k-th runs of all circuits in
 //    one launch.  Circuits that finish answer their queries and stay resident;
 //    their slots take new arrivals at the next step, while larger circuits
-//    are still in flight.
-// Both policies compute every amplitude with the same arithmetic in the same
-// order, so a query's answer does not depend on the policy or on its batch.
+//    are still in flight;
+//  * network: continuous batching as above, but each amplitude query is first
+//    planned as the tensor network <x|C|0> (best of four greedy orders whose
+//    intermediates stay within rank 28) and takes the cheaper route
+//    by estimated time: the state vector, or its network, contracted together
+//    with the other network queries admitted in the same step (one grouped
+//    launch per step index, GEMM steps in FP64 or split precision).  With
+//    QTNH_SERVE_ALL_NETWORK every amplitude query whose plan fits takes the
+//    network route.
+// The state-vector route computes every value with the same arithmetic in the
+// same order under every policy, and a network's value does not depend on the
+// networks it runs with, so a query's answer never depends on its batch.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -129,6 +140,55 @@ qtnh_status make_circuit(const Key &k, CircuitHandle &h) {
   return qtnh_circuit_random(h.c, k.depth, k.seed);
 }
 
+struct NetHandle {
+  qtnh_net_t net = nullptr;
+  NetHandle() = default;
+  NetHandle(const NetHandle &) = delete;
+  ~NetHandle() {
+    if (net) qtnh_net_destroy(net);
+  }
+};
+
+// Route estimates (seconds), from measurements on one MI355X.  A fused run
+// reads and writes the state once at about 2.6 TB/s effective (its FP64 op
+// work sits above the HBM time: rank 30, 12.9 ms per 16 GiB state) and costs
+// at least ~10 us; a network step costs ~4 us of launch and host work, its
+// flops run at ~55 TFLOP/s in FP64 (~100 TFLOP/s where split-precision GEMM
+// steps fill the GPU) and its bytes at ~3 TB/s.
+double state_route_s(int n, int nruns) {
+  return (double)nruns * std::max(10e-6, 32.0 * (double)(1ull << n) / 2.6e12);
+}
+double network_route_s(double steps, double flops, double bytes, bool split) {
+  return steps * 4e-6 + flops / (split ? 100e12 : 55e12) + bytes / 3e12;
+}
+
+// Plans the amplitude network <basis|C|0> of query q (best of four greedy
+// orders whose intermediates stay within rank 28, 4 GiB).  *use_net is set
+// when the network route is taken; *bytes gets its device bytes.
+qtnh_status plan_amplitude(qtnh_ctx_t ctx, const qtnh_query &q, qtnh_circuit_t c, bool split, bool all,
+                           NetHandle &h, bool *use_net, uint64_t *bytes) {
+  *use_net = false;
+  const int n = q.n_qubits;
+  std::vector<uint8_t> bits(n);
+  for (int j = 0; j < n; ++j) bits[j] = (uint8_t)((q.basis >> (n - 1 - j)) & 1);
+  qtnh_status st = qtnh_net_create(&h.net);
+  if (!st) st = qtnh_net_from_circuit(h.net, c, bits.data());
+  if (st) return st;
+  st = qtnh_net_plan_best(h.net, q.seed, 28, 4, nullptr);
+  if (st == QTNH_EUNSUPPORTED) return QTNH_OK;  // no order within the cap: state vector
+  if (st) return st;
+  if (split) st = qtnh_net_set_precision(h.net, QTNH_PREC_SPLIT);
+  uint64_t steps = 0, dev = 0;
+  double flops = 0, nbytes = 0;
+  if (!st) st = qtnh_net_info(h.net, nullptr, &steps, nullptr, &flops, &nbytes, &dev);
+  int nruns = 0;
+  if (!st && !all) st = circuit_run_count(ctx, c, &nruns);
+  if (st) return st;
+  *use_net = all || network_route_s((double)steps, flops, nbytes, split) < state_route_s(n, nruns);
+  *bytes = dev;
+  return QTNH_OK;
+}
+
 // A circuit in flight: its state, its next fused run and the queries waiting on it.
 struct Flight {
   Key key{0, 0, 0};
@@ -149,7 +209,12 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
   qtnh_status st = enter_ctx(ctx);
   if (st) return st;
   if (nq < 0 || (nq > 0 && (!queries || !results))) return set_error(ctx, QTNH_EINVAL, "serve: bad arguments");
-  if (policy != QTNH_SERVE_SEQUENTIAL && policy != QTNH_SERVE_BATCHED)
+  const int base = policy & 0xFF;
+  const bool split = (policy & QTNH_SERVE_SPLIT) != 0, all_net = (policy & QTNH_SERVE_ALL_NETWORK) != 0;
+  if ((base != QTNH_SERVE_SEQUENTIAL && base != QTNH_SERVE_BATCHED && base != QTNH_SERVE_SEQUENTIAL_FUSED &&
+       base != QTNH_SERVE_NETWORK) ||
+      (policy & ~(0xFF | QTNH_SERVE_SPLIT | QTNH_SERVE_ALL_NETWORK)) ||
+      ((split || all_net) && base != QTNH_SERVE_NETWORK))
     return set_error(ctx, QTNH_EINVAL, "serve: unknown policy %d", policy);
   std::vector<int> order(nq);
   for (int i = 0; i < nq; ++i) {
@@ -174,6 +239,14 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
   int step = 0;      // batched policy: steps run so far
   std::vector<Flight> flight;
   uint64_t inflight_bytes = 0;
+  // Network-route queries admitted for the next step, and their device bytes.
+  struct NetQuery {
+    int query;
+    std::unique_ptr<NetHandle> h;
+    uint64_t bytes;
+  };
+  std::vector<NetQuery> net_wait;
+  uint64_t net_bytes = 0;
   struct FlightGuard {  // frees states still in flight on an error return
     std::vector<Flight> &f;
     ~FlightGuard() {
@@ -184,13 +257,13 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
   while (done < (size_t)nq) {
     double t = now_s(t0);
     while (next < order.size() && queries[order[next]].arrival_s <= t) pending.push_back(order[next++]);
-    if (pending.empty() && flight.empty()) {
+    if (pending.empty() && flight.empty() && net_wait.empty()) {
       if (next >= order.size()) return set_error(ctx, QTNH_EINVAL, "serve: internal: no work left");
       const double wait = queries[order[next]].arrival_s - t;
       if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
       continue;
     }
-    if (policy == QTNH_SERVE_SEQUENTIAL) {
+    if (base == QTNH_SERVE_SEQUENTIAL || base == QTNH_SERVE_SEQUENTIAL_FUSED) {
       const int i = pending.front();
       pending.erase(pending.begin());
       const auto &q = queries[i];
@@ -203,7 +276,8 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
       CircuitHandle h;
       st = make_circuit({q.n_qubits, q.depth, q.seed}, h);
       if (!st) st = init_state(ctx, buf, q.n_qubits);
-      if (!st) st = qtnh_circuit_run(ctx, h.c, buf, QTNH_RUN_PER_GATE);
+      if (!st)
+        st = qtnh_circuit_run(ctx, h.c, buf, base == QTNH_SERVE_SEQUENTIAL ? QTNH_RUN_PER_GATE : QTNH_RUN_FUSED);
       if (!st) st = answer(ctx, q, buf, r);
       (void)hipFree(buf);
       if (st) return st;
@@ -243,16 +317,35 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
         pending.erase(pending.begin());
         continue;
       }
+      std::shared_ptr<CircuitHandle> ch = std::make_shared<CircuitHandle>();
+      st = make_circuit(key, *ch);
+      if (st) return st;
+      if (base == QTNH_SERVE_NETWORK && q.kind == QTNH_QUERY_AMPLITUDE) {
+        std::unique_ptr<NetHandle> nh(new NetHandle);
+        bool use_net = false;
+        uint64_t nb = 0;
+        st = plan_amplitude(ctx, q, ch->c, split, all_net, *nh, &use_net, &nb);
+        if (st) return st;
+        if (use_net) {
+          if (!srv.make_room(nb + inflight_bytes + net_bytes)) break;  // waits for memory
+          results[i].start_s = now_s(t0);
+          results[i].batch = step;
+          results[i].route = 1;
+          net_wait.push_back({i, std::move(nh), nb});
+          net_bytes += nb;
+          pending.erase(pending.begin());
+          continue;
+        }
+      }
       const uint64_t bytes = 16ull << q.n_qubits;
-      if (!srv.make_room(bytes + inflight_bytes)) break;  // waits for a slot to free
+      if (!srv.make_room(bytes + inflight_bytes + net_bytes)) break;  // waits for a slot to free
       Flight f;
       f.key = key;
       f.bytes = bytes;
       st = check_hip(ctx, hipMalloc(&f.buf, bytes), "serve state alloc");
       if (st) return st;
-      f.h = std::make_shared<CircuitHandle>();
-      st = make_circuit(key, *f.h);
-      if (!st) st = init_state(ctx, f.buf, key.n);
+      f.h = ch;
+      st = init_state(ctx, f.buf, key.n);
       if (!st) st = circuit_run_count(ctx, f.h->c, &f.nruns);
       if (st) {
         (void)hipFree(f.buf);
@@ -265,11 +358,37 @@ extern "C" qtnh_status qtnh_serve(qtnh_ctx_t ctx, int nq, const qtnh_query *quer
       flight.push_back(std::move(f));
       pending.erase(pending.begin());
     }
-    if (flight.empty()) {
+    if (flight.empty() && net_wait.empty()) {
       // Nothing in flight and nothing admitted: every pending query fits an
       // empty device (checked above), so only future arrivals remain.
       if (!pending.empty()) return set_error(ctx, QTNH_ENOMEM, "serve: admission stalled");
       continue;
+    }
+    // The network-route queries of this step, contracted together.
+    if (!net_wait.empty()) {
+      std::vector<qtnh_net_t> nets;
+      std::vector<std::vector<double>> outs(net_wait.size(), std::vector<double>(2, 0.0));
+      std::vector<double *> op;
+      for (size_t k = 0; k < net_wait.size(); ++k) {
+        nets.push_back(net_wait[k].h->net);
+        op.push_back(outs[k].data());
+      }
+      int waves = 0;
+      st = qtnh_net_run_batch(ctx, (int)nets.size(), nets.data(), op.data(), srv.budget, &waves);
+      if (st) return st;
+      for (size_t k = 0; k < net_wait.size(); ++k) {
+        auto &r = results[net_wait[k].query];
+        r.value[0] = outs[k][0];
+        r.value[1] = outs[k][1];
+        r.done_s = now_s(t0);
+        ++done;
+      }
+      net_wait.clear();
+      net_bytes = 0;
+      if (flight.empty()) {
+        ++step;
+        continue;
+      }
     }
     // One step: every circuit in flight advances (states of up to 22 qubits
     // to the end, 23-25 qubits by up to 8 runs, larger ones by one run: a step

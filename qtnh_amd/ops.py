@@ -478,6 +478,13 @@ class Network:
         check(lib().qtnh_net_plan(self._h, seed, max_rank))
         return self
 
+    def plan_best(self, seed=0, max_rank=26, tries=4):
+        """The fewest-flop plan among `tries` seeded greedy orders within max_rank;
+        returns the chosen seed."""
+        chosen = ctypes.c_uint64(0)
+        check(lib().qtnh_net_plan_best(self._h, seed, max_rank, tries, ctypes.byref(chosen)))
+        return chosen.value
+
     def set_precision(self, precision):
         """_lib.PREC_FP64 (default) or _lib.PREC_SPLIT for the GEMM-route steps."""
         check(lib().qtnh_net_set_precision(self._h, precision))
@@ -562,7 +569,9 @@ def run_networks(nets, budget_bytes=1 << 40, device=None):
 def serve(queries, budget_bytes, policy=_lib.SERVE_BATCHED, device=None):
     """Run the serving loop on a list of dicts (n_qubits, depth, seed, kind,
     basis, arrival_s); returns one dict per query with start/done times,
-    latency (done - arrival), value, zsum, batch and cached."""
+    latency (done - arrival), value, zsum, batch, cached and route.  policy:
+    one of _lib.SERVE_*, with _lib.SERVE_NETWORK optionally or-ed with
+    _lib.SERVE_SPLIT / _lib.SERVE_ALL_NETWORK."""
     n = len(queries)
     qs = (_lib.Query * max(n, 1))()
     for i, q in enumerate(queries):
@@ -575,7 +584,8 @@ def serve(queries, budget_bytes, policy=_lib.SERVE_BATCHED, device=None):
     for i in range(n):
         r = rs[i]
         out.append({"start_s": r.start_s, "done_s": r.done_s, "latency_s": r.done_s - queries[i]["arrival_s"],
-                    "value": (r.value[0], r.value[1]), "zsum": r.zsum, "batch": r.batch, "cached": r.cached})
+                    "value": (r.value[0], r.value[1]), "zsum": r.zsum, "batch": r.batch, "cached": r.cached,
+                    "route": "network" if r.route else "state"})
     return out
 
 

@@ -2,6 +2,7 @@
 // (random networks, QFT known answer) plus the networks the reference never
 // ran (tn3/tn4, c2-network.cpp:49,90 reuse tn2) and a full-state QFT check.
 #include <cmath>
+#include <cstring>
 
 #include "golden_gen.hpp"
 #include "harness.hpp"
@@ -129,6 +130,72 @@ TEST_CASE("qft-distributed-qubits") {
       auto order = build_qft(tn, n, dq, false);
       auto tp = tn.extract(tn.contractAll(order));
       REQUIRE(tp->totSize() == (std::size_t)1 << n);
+      REQUIRE(utils::equal(tp->at(utils::i_to_idxs(0, tp->totDims())), 1.0, 1e-12));
+    }
+}
+
+static bool same_bits(const Tensor& a, const Tensor& b) {
+  if (a.totDims() != b.totDims()) return false;
+  for (std::size_t k = 0; k < a.totSize(); ++k) {
+    const tel x = a[k], y = b[k];
+    if (std::memcmp(&x, &y, sizeof(tel)) != 0) return false;
+  }
+  return true;
+}
+
+TEST_CASE("network-one-device-plan-equals-bond-by-bond") {
+  // contractAll(order) runs the whole order as one device plan; contracting
+  // the same bonds one contractBond call at a time (a one-step plan each,
+  // results passed on through device storage) gives the same bits.
+  for (auto* group : {&golden::tn3_vals, &golden::tn4_vals, &golden::tn4c_vals})
+    for (auto& nv : *group) {
+      TensorNetwork t1, t2;
+      std::vector<qtnh::uint> i1, i2, o1, o2;
+      for (auto& t : nv.tensors) {
+        i1.push_back(t1.make<DenseTensor>(ENV, tidx_tup{}, t.dims, std::vector<tel>(t.els.begin(), t.els.end())));
+        i2.push_back(t2.make<DenseTensor>(ENV, tidx_tup{}, t.dims, std::vector<tel>(t.els.begin(), t.els.end())));
+      }
+      for (auto& b : nv.bonds) {
+        o1.push_back(t1.addBond(i1.at(b.t1), i1.at(b.t2), b.wires));
+        o2.push_back(t2.addBond(i2.at(b.t1), i2.at(b.t2), b.wires));
+      }
+      auto whole = t1.extract(t1.contractAll(o1));
+      REQUIRE(t1.lastPlanSteps() == nv.tensors.size() - 1);
+      qtnh::uint last = 0;
+      if (group == &golden::tn4c_vals) {
+        // a ring: after three steps the closing bond joins the same two
+        // tensors as the third, and the order contracts both together
+        REQUIRE(o2.size() == 4);
+        for (auto ids : {std::vector<qtnh::uint>{o2[0]}, {o2[1]}, {o2[2], o2[3]}}) {
+          last = t2.contractAll(ids);
+          REQUIRE(t2.lastPlanSteps() == 1);
+        }
+      } else {
+        for (auto bid : o2) {
+          last = t2.contractBond(bid);
+          REQUIRE(t2.lastPlanSteps() == 1);
+        }
+      }
+      auto stepwise = t2.extract(last);
+      REQUIRE(same_bits(*whole, *stepwise));
+    }
+}
+
+TEST_CASE("qft-network-steps") {
+  // Every tensor but identities contracted in place costs one step of the plan.
+  for (unsigned n : {3u, 5u})
+    for (unsigned dq : {0u, 1u}) {
+      TensorNetwork tn;
+      auto order = build_qft(tn, n, dq, true);
+      std::size_t tensors = tn.tensorIDs().size(), idens = 0;
+      for (auto id : tn.tensorIDs()) idens += tn.tensor(id)->type() == TT::idenTensor;
+      auto id = tn.contractAll(order);
+      REQUIRE(tn.tensorIDs().size() == 1 && tn.tensorIDs()[0] == id);
+      REQUIRE(tn.bondIDs().empty());
+      REQUIRE(tn.lastPlanSteps() <= tensors - 1);
+      REQUIRE(tn.lastPlanSteps() + idens >= tensors - 1);
+      if (dq == 0) REQUIRE(tn.lastPlanSteps() == tensors - 1);
+      auto tp = tn.extract(id);
       REQUIRE(utils::equal(tp->at(utils::i_to_idxs(0, tp->totDims())), 1.0, 1e-12));
     }
 }

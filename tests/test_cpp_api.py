@@ -6,7 +6,7 @@ import subprocess
 import pytest
 
 BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bin")
-TESTS = ["test_indexing", "test_tensor", "test_network"]
+TESTS = ["test_indexing", "test_tensor", "test_network", "test_print"]
 
 
 def _run(name, *args):

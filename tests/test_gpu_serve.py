@@ -98,3 +98,90 @@ def test_oversized_query_rejected():
     qs = [{"n_qubits": 20, "depth": 1, "seed": 1, "kind": 0, "basis": 0, "arrival_s": 0.0}]
     with pytest.raises(_lib.QtnhError):
         ops.serve(qs, budget_bytes=1 << 20)
+
+
+def test_sequential_fused_equals_solo():
+    """The one-at-a-time fused baseline: fused runs, one query at a time."""
+    qs = ops.poisson_queries(16, rate=0, seed=13, qubits=(3, 14), depth=(2, 8))
+    rs = ops.serve(qs, budget_bytes=1 << 30, policy=_lib.SERVE_SEQUENTIAL_FUSED)
+    check_answers(qs, rs)
+    assert sorted(r["batch"] for r in rs) == list(range(len(qs)))
+
+
+# Amplitude queries large enough that some network steps are GEMMs wide enough
+# for split precision, plus <Z> queries (always the state vector).
+NET_QUERIES = [(22, 8, 200), (24, 7, 202), (24, 8, 201), (20, 8, 202), (24, 6, 202), (12, 10, 106)]
+
+# Stated tolerances of a network-route amplitude against the state vector's
+# (a unit-norm state): FP64 and split precision.
+TOL_FP64 = 1e-12
+TOL_SPLIT = 1e-11  # measured at most 7e-14 on these queries
+
+
+def net_queries():
+    qs = []
+    rng = np.random.default_rng(41)
+    for n, d, s in NET_QUERIES:
+        qs.append({"n_qubits": n, "depth": d, "seed": s, "kind": _lib.QUERY_AMPLITUDE,
+                   "basis": int(rng.integers(0, 2 ** n)), "arrival_s": 0.0})
+    qs.append({"n_qubits": 16, "depth": 6, "seed": 107, "kind": _lib.QUERY_EXPECT_Z, "basis": 0, "arrival_s": 0.0})
+    qs.append({"n_qubits": 20, "depth": 4, "seed": 108, "kind": _lib.QUERY_EXPECT_Z, "basis": 0, "arrival_s": 0.0})
+    return qs
+
+
+def amp_err(r, want):
+    return abs(complex(*r["value"]) - complex(*want))
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_network_route_alone_and_in_the_mix(split):
+    """Every amplitude query whose plan fits rank 28 takes the network route: its answer is bitwise the same served alone and in the
+    mix, and within the stated tolerance of the state-vector amplitude; <Z>
+    queries take the state route and equal the per-gate solo run bitwise."""
+    pol = _lib.SERVE_NETWORK | _lib.SERVE_ALL_NETWORK | (_lib.SERVE_SPLIT if split else 0)
+    qs = net_queries()
+    rs = ops.serve(qs, budget_bytes=8 << 30, policy=pol)
+    tol = TOL_SPLIT if split else TOL_FP64
+    worst = 0.0
+    nets = 0
+    for q, r in zip(qs, rs):
+        alone = ops.serve([dict(q)], budget_bytes=8 << 30, policy=pol)[0]
+        assert alone["route"] == r["route"]
+        assert np.array_equal(np.array(alone["value"]).view(np.uint64), np.array(r["value"]).view(np.uint64)), q
+        want, _ = solo(q)
+        if r["route"] == "network":
+            nets += 1
+            e = amp_err(r, want)
+            worst = max(worst, e)
+            assert e <= tol, (q, e)
+        else:
+            assert np.array_equal(np.array(r["value"]).view(np.uint64), np.array(want).view(np.uint64)), q
+    print("network answers: %d, worst |error| %.3e (split=%s)" % (nets, worst, split), flush=True)
+    assert nets >= 4
+
+
+def test_split_network_answers_differ_from_fp64_within_tolerance():
+    """Split precision really runs on the wide GEMM steps: some answers change,
+    each within the stated split tolerance of the FP64 network answer."""
+    qs = [q for q in net_queries() if q["kind"] == _lib.QUERY_AMPLITUDE]
+    base = _lib.SERVE_NETWORK | _lib.SERVE_ALL_NETWORK
+    r64 = ops.serve(qs, budget_bytes=8 << 30, policy=base)
+    rsp = ops.serve(qs, budget_bytes=8 << 30, policy=base | _lib.SERVE_SPLIT)
+    diffs = [amp_err(a, b["value"]) for a, b in zip(rsp, r64) if a["route"] == "network"]
+    print("split vs fp64 network answers:", ["%.2e" % d for d in diffs], flush=True)
+    assert any(d > 0 for d in diffs)
+    assert max(diffs) <= TOL_SPLIT
+
+
+def test_network_policy_routes_by_cost():
+    """Without the all-network flag the cheaper route is taken; state-route
+    answers equal the solo runs bitwise, network-route answers are within the
+    FP64 tolerance."""
+    qs = ops.poisson_queries(20, rate=0, seed=17, qubits=(6, 22), depth=(2, 10), amp_fraction=0.8)
+    rs = ops.serve(qs, budget_bytes=8 << 30, policy=_lib.SERVE_NETWORK)
+    for q, r in zip(qs, rs):
+        want, _ = solo(q)
+        if r["route"] == "network":
+            assert amp_err(r, want) <= TOL_FP64, q
+        else:
+            assert np.array_equal(np.array(r["value"]).view(np.uint64), np.array(want).view(np.uint64)), q
