@@ -207,6 +207,47 @@ def bench_c1(torch, ops, _lib):
     pg = torch.view_as_real(res["per_gate"])
     out["bitwise_equal"] = bool(torch.equal(pg, torch.view_as_real(res["fused"])) and
                                 torch.equal(pg, torch.view_as_real(res["fused_tuned"])))
+    out["cpu"] = cpu_qft(n, res["per_gate"].cpu().numpy(), x0.cpu().numpy())
+    return out
+
+
+def cpu_qft(n, gpu_out, psi0):
+    """QFT-n on the host: the C oracle (one thread, the reference's gate
+    order and arithmetic) and the same gates through the oracle's threaded
+    gate and diagonal functions; per-circuit latency percentiles.  The host
+    results are compared bitwise with the GPU's."""
+    import numpy as np
+
+    import oracle
+    phases = np.zeros(n, dtype=np.complex128)
+    for m in range(1, n):
+        p = math.pi / 2.0 ** m
+        phases[m] = complex(math.cos(p), math.sin(p))
+    h = 2.0 ** -0.5
+    H = np.array([h, h, h, -h], dtype=np.complex128)
+    dims = [2] * n
+    threads = _host_threads()
+    out = {"threads": threads, "host_cpu": _cpu_model()}
+    one, mt = [], []
+    for _ in range(20):
+        oracle.qft(psi0, n, phases)
+    for _ in range(200):
+        t0 = time.perf_counter()
+        ref = oracle.qft(psi0, n, phases)
+        one.append((time.perf_counter() - t0) * 1e6)
+    for rep in range(60):
+        x = psi0.copy()
+        t0 = time.perf_counter()
+        for i in range(n):
+            oracle.gate_apply_mt(x, dims, [i], H, threads, inplace=True)
+            for j in range(i + 1, n):
+                oracle.diag_apply_mt(x, dims, [i, j], np.array([1, 1, 1, phases[j - i]]), threads, inplace=True)
+        if rep >= 10:
+            mt.append((time.perf_counter() - t0) * 1e6)
+    out["one_core"] = {"p50_us": round(_pct(one, 50), 1), "p99_us": round(_pct(one, 99), 1)}
+    out["threaded"] = {"p50_us": round(_pct(mt, 50), 1), "p99_us": round(_pct(mt, 99), 1)}
+    out["bitwise_equal_to_gpu"] = bool(np.array_equal(ref.view(np.uint64), gpu_out.view(np.uint64)) and
+                                       np.array_equal(x.view(np.uint64), gpu_out.view(np.uint64)))
     return out
 
 
@@ -353,46 +394,75 @@ def bench_c4(torch, ops, _lib, count, max_rank, seed):
             and all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(solo, replay))}
 
 
+def _serve_stats(count, r):
+    lat = [x["latency_s"] * 1e3 for x in r]
+    return {"p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
+            "throughput_qps": round(count / max(x["done_s"] for x in r), 2),
+            "network_route": sum(1 for x in r if x["route"] == "network")}
+
+
 def bench_c5(torch, ops, _lib, count, rate, seed):
     """Serving: Poisson arrivals, circuits of 10..28 qubits and depth 10..30
-    (random SU(2) + CZ), half amplitude and half <Z> queries; sequential
-    one-launch-per-gate FP64 vs batched fused FP64 with resident states, at the
-    arrival rate and at saturation (every query arriving at once)."""
+    (random SU(2) + CZ), half amplitude and half <Z> queries, at the arrival
+    rate and at saturation (every query arriving at once).  Policies:
+    sequential (one launch per gate) and sequential fused runs, one query at a
+    time; batched (continuous batching of fused runs, resident states); network
+    (batched, each amplitude query on the cheaper route: state vector or its
+    tensor network), in FP64 and with split-precision GEMM steps.  Then every
+    amplitude query whose network plan fits rank 28 forced onto the network
+    route, FP64 vs split, with the per-result error of split against FP64."""
     import numpy as np
     qs = ops.poisson_queries(count, rate, seed, qubits=(10, 28), depth=(10, 30))
     out = {"queries": count, "rate_per_s": rate, "qubits": [10, 28], "depth": [10, 30]}
-    res = {}
-    for name, pol in (("sequential", _lib.SERVE_SEQUENTIAL), ("batched", _lib.SERVE_BATCHED)):
-        t0 = time.perf_counter()
-        r = ops.serve(qs, budget_bytes=200 << 30, policy=pol)
-        wall = time.perf_counter() - t0
-        lat = [x["latency_s"] * 1e3 for x in r]
-        res[name] = r
-        out[name] = {"p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
-                     "throughput_qps": round(count / max(x["done_s"] for x in r), 2), "wall_s": round(wall, 2)}
-    out["p99_speedup"] = round(out["sequential"]["p99_ms"] / out["batched"]["p99_ms"], 2)
-    out["throughput_speedup"] = round(out["batched"]["throughput_qps"] / out["sequential"]["throughput_qps"], 2)
+    pols = (("sequential", _lib.SERVE_SEQUENTIAL), ("sequential_fused", _lib.SERVE_SEQUENTIAL_FUSED),
+            ("batched", _lib.SERVE_BATCHED), ("network", _lib.SERVE_NETWORK),
+            ("network_split", _lib.SERVE_NETWORK | _lib.SERVE_SPLIT))
     same = lambda a, b: np.array_equal(np.array(a["value"]).view(np.uint64),  # noqa: E731
                                        np.array(b["value"]).view(np.uint64)) and a["zsum"] == b["zsum"]
-    out["answers_bitwise_equal"] = all(same(a, b) for a, b in zip(res["sequential"], res["batched"]))
-    # Saturation: the same queries all arriving at t = 0, so throughput is the
-    # service rate of each policy rather than the arrival rate.
     sat = [dict(q, arrival_s=0.0) for q in qs]
-    sres = {}
+    res = {}
     out["saturation"] = {}
-    for name, pol in (("sequential", _lib.SERVE_SEQUENTIAL), ("batched", _lib.SERVE_BATCHED)):
+    for label, qlist, dst in (("rate", qs, out), ("saturation", sat, out["saturation"])):
+        for name, pol in pols:
+            t0 = time.perf_counter()
+            r = ops.serve(qlist, budget_bytes=200 << 30, policy=pol)
+            dst[name] = _serve_stats(count, r)
+            dst[name]["wall_s"] = round(time.perf_counter() - t0, 2)
+            res[(label, name)] = r
+        for base in ("sequential", "sequential_fused"):
+            dst["throughput_speedup_vs_" + base] = round(
+                dst["batched"]["throughput_qps"] / dst[base]["throughput_qps"], 2)
+            dst["p99_speedup_vs_" + base] = round(dst[base]["p99_ms"] / dst["batched"]["p99_ms"], 2)
+        # State-route answers are identical under every policy.
+        dst["answers_bitwise_equal"] = all(
+            same(a, b) for name in ("sequential_fused", "batched") for a, b in
+            zip(res[(label, "sequential")], res[(label, name)])) and all(
+            same(a, b) for name in ("network", "network_split") for a, b in
+            zip(res[(label, "batched")], res[(label, name)]) if b["route"] == "state")
+    out["answers_equal_at_rate_and_saturation"] = all(
+        same(a, b) for a, b in zip(res[("rate", "batched")], res[("saturation", "batched")]))
+    # Forced network route at saturation: FP64 vs split precision.
+    allnet = _lib.SERVE_NETWORK | _lib.SERVE_ALL_NETWORK
+    forced = {}
+    for name, pol in (("fp64", allnet), ("split", allnet | _lib.SERVE_SPLIT)):
         r = ops.serve(sat, budget_bytes=200 << 30, policy=pol)
-        lat = [x["latency_s"] * 1e3 for x in r]
-        sres[name] = r
-        out["saturation"][name] = {"p50_ms": round(_pct(lat, 50), 2), "p99_ms": round(_pct(lat, 99), 2),
-                                   "throughput_qps": round(count / max(x["done_s"] for x in r), 2)}
-    out["saturation"]["throughput_speedup"] = round(
-        out["saturation"]["batched"]["throughput_qps"] / out["saturation"]["sequential"]["throughput_qps"], 2)
-    out["saturation"]["p99_speedup"] = round(
-        out["saturation"]["sequential"]["p99_ms"] / out["saturation"]["batched"]["p99_ms"], 2)
-    out["saturation"]["answers_bitwise_equal"] = all(
-        same(a, b) for a, b in zip(sres["sequential"], sres["batched"])) and all(
-        same(a, b) for a, b in zip(sres["batched"], res["batched"]))
+        forced[name] = r
+        out["saturation"]["all_network_" + name] = _serve_stats(count, r)
+    sv = res[("saturation", "batched")]
+    errs, rels, errs64 = [], [], []
+    for a, b, c in zip(forced["split"], forced["fp64"], sv):
+        if a["route"] != "network":
+            continue
+        va, vb, vc = complex(*a["value"]), complex(*b["value"]), complex(*c["value"])
+        errs.append(abs(va - vb))
+        rels.append(abs(va - vb) / max(abs(vb), 1e-300))
+        errs64.append(abs(vb - vc))
+    out["saturation"]["all_network_errors"] = {
+        "network_answers": len(errs),
+        "split_vs_fp64_network_abs": {"max": max(errs, default=0.0), "median": float(np.median(errs)) if errs else 0.0},
+        "split_vs_fp64_network_rel": {"max": max(rels, default=0.0), "median": float(np.median(rels)) if rels else 0.0},
+        "fp64_network_vs_state_vector_abs_max": max(errs64, default=0.0),
+        "per_result_abs": [float("%.3e" % e) for e in errs]}
     return out
 
 
