@@ -278,8 +278,9 @@ qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t o
  * the context's workspace, the last step writes out directly.  out must not
  * alias a leaf. */
 qtnh_status qtnh_net_run_device(qtnh_ctx_t ctx, qtnh_net_t net, void *out);
-/* Contracts planned networks together: step k of every network in one grouped
- * launch (small binary contractions) plus the large steps in turn.  Networks
+/* Contracts planned networks together: step k of every network in grouped
+ * launches (small binary contractions; small FP64 GEMM-route steps as grouped
+ * operand permutes, GEMM tiles and split-K sums) plus the large steps in turn.  Networks
  * are admitted in consecutive waves whose device blocks fit budget_bytes.
  * outs[i] (host) receives network i's result; results are bit-identical to
  * qtnh_net_run.  *waves (may be NULL) gets the number of waves.  Blocks. */
@@ -297,6 +298,12 @@ qtnh_status qtnh_net_batch_create(qtnh_ctx_t ctx, int count, const qtnh_net_t *n
 qtnh_status qtnh_net_batch_set_leaf(qtnh_ctx_t ctx, qtnh_net_batch_t b, int net, int leaf, const double *data);
 /* Contracts every network of the batch; outs[i] (host) receives network i's result.  Blocks. */
 qtnh_status qtnh_net_batch_run(qtnh_ctx_t ctx, qtnh_net_batch_t b, double *const *outs);
+/* How the batch's steps run (any may be NULL): small direct steps in one
+ * grouped launch per step index; small FP64 GEMM-route steps in the step
+ * index's grouped permute / GEMM / split-K launches; the rest one launch
+ * sequence each. */
+qtnh_status qtnh_net_batch_stats(qtnh_net_batch_t b, uint64_t *grouped_direct, uint64_t *grouped_gemm,
+                                 uint64_t *single);
 qtnh_status qtnh_net_batch_destroy(qtnh_net_batch_t b);
 
 /* ---- serving loop (no reference counterpart): queries on the output states of

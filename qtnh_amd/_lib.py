@@ -87,6 +87,7 @@ SIGNATURES = {
     "qtnh_net_add_tensor_device": [_p, _i, _u64p, ctypes.POINTER(ctypes.c_int64), _p, ctypes.POINTER(_i)],
     "qtnh_net_plan_steps": [_p, _i, _p],
     "qtnh_net_plan_best": [_p, _u64, _i, _i, _u64p],
+    "qtnh_net_batch_stats": [_p, _u64p, _u64p, _u64p],
     "qtnh_net_run_device": [_p, _p, _p],
     "qtnh_net_from_circuit": [_p, _p, ctypes.POINTER(ctypes.c_uint8)],
     "qtnh_net_plan": [_p, _u64, _i],

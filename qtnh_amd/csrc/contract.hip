@@ -600,22 +600,18 @@ static uint64_t splitk_chunks(uint64_t MN, uint64_t K) {
 // the natural result [open1, open2] is permuted to the placement.  Operands
 // already in those layouts are used in place.  Workspace: A', B', natural C,
 // then split scratch, split-K partials or GEMM depth-chunk partials.
-struct GemmLayout {
-  bool splitk, split;
-  std::vector<uint16_t> pA, pB, pC;
-  bool idA = true, idB = true, idC = true;
-  uint64_t kchunks, zchunks, sbytes, bytes;
-};
-
 static uint64_t pad_c128(uint64_t n) { return ((n * sizeof(c128)) + 255) & ~255ull; }
 
-static GemmLayout gemm_layout(const Plan &P, int precision) {
-  GemmLayout L;
+static void gemm_layout(const Plan &P, const uint64_t *dA, const uint64_t *dB, int precision, GemmRoute &L) {
   L.splitk = use_splitk(P);
   L.split = precision == QTNH_PREC_SPLIT && !L.splitk;
-  L.pA.resize(P.rA);
-  L.pB.resize(P.rB);
-  L.pC.resize(P.rC);
+  L.M = P.M;
+  L.N = P.N;
+  L.K = P.K;
+  L.nC = P.nC;
+  L.pA.assign(P.rA, 0);
+  L.pB.assign(P.rB, 0);
+  L.pC.assign(P.rC, 0);
   int q = 0;
   for (int i : P.open1) L.pA[i] = (uint16_t)q++;
   for (int i : P.cA) L.pA[i] = (uint16_t)q++;
@@ -627,21 +623,39 @@ static GemmLayout gemm_layout(const Plan &P, int precision) {
     for (int i : P.cB) L.pB[i] = (uint16_t)q++;
     for (int i : P.open2) L.pB[i] = (uint16_t)q++;
   }
+  L.idA = L.idB = L.idC = true;
   for (int i = 0; i < P.rA; ++i) L.idA = L.idA && L.pA[i] == i;
   for (int i = 0; i < P.rB; ++i) L.idB = L.idB && L.pB[i] == i;
   for (int i = 0; i < P.rC; ++i) {
     L.pC[i] = (uint16_t)P.pos[i];
     L.idC = L.idC && P.pos[i] == i;
   }
+  L.dn.clear();
+  for (int i : P.open1) L.dn.push_back(dA[i]);
+  for (int i : P.open2) L.dn.push_back(dB[i]);
   L.kchunks = L.splitk ? splitk_chunks(P.M * P.N, P.K) : 0;
   L.zchunks = (L.split || L.splitk) ? 1 : zgemm_chunks(P.M, P.N, P.K);
   L.sbytes = L.split         ? ((split_scratch_bytes(P.M, P.N, P.K) + 255) & ~255ull)
              : L.splitk      ? pad_c128(P.M * P.N * L.kchunks)
              : L.zchunks > 1 ? pad_c128(P.M * P.N * L.zchunks)
                              : 0;
-  L.bytes = (L.idA ? 0 : pad_c128(P.M * P.K)) + (L.idB ? 0 : pad_c128(P.K * P.N)) + (L.idC ? 0 : pad_c128(P.nC)) +
-            L.sbytes;
-  return L;
+  uint64_t off = 0;
+  L.offA = off;
+  off += L.idA ? 0 : pad_c128(P.M * P.K);
+  L.offB = off;
+  off += L.idB ? 0 : pad_c128(P.K * P.N);
+  L.offC = off;
+  off += L.idC ? 0 : pad_c128(P.nC);
+  L.offS = off;
+  L.bytes = off + L.sbytes;
+}
+
+bool gemm_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                const uint16_t *r1, const uint16_t *r2, int precision, GemmRoute *out) {
+  Plan P = make_plan(rA, dA, rB, dB, nw, wires, r1, r2);
+  if (use_direct(P)) return false;
+  gemm_layout(P, dA, dB, precision, *out);
+  return true;
 }
 
 ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
@@ -652,7 +666,12 @@ ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t 
   R.N = P.N;
   R.K = P.K;
   R.direct = use_direct(P);
-  R.ws_bytes = R.direct ? 0 : gemm_layout(P, precision).bytes;
+  R.ws_bytes = 0;
+  if (!R.direct) {
+    GemmRoute L;
+    gemm_layout(P, dA, dB, precision, L);
+    R.ws_bytes = L.bytes;
+  }
   return R;
 }
 
@@ -665,59 +684,46 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     if (err_out) *err_out = 0.0;
     return launch_contract_direct(ctx, A, rA, dA, B, rB, dB, nw, wires, r1, r2, C);
   }
-  const GemmLayout L = gemm_layout(P, precision);
-  const uint64_t bytes = L.bytes;
+  GemmRoute L;
+  gemm_layout(P, dA, dB, precision, L);
   char *ws = nullptr;
-  if (bytes && ext_ws) {
+  if (L.bytes && ext_ws) {
     // A caller-provided region (batches, captured graphs): never reallocated here.
-    if (ext_ws_bytes < bytes) return set_error(ctx, QTNH_EINVAL, "contract: workspace of %llu bytes, need %llu",
-                                               (unsigned long long)ext_ws_bytes, (unsigned long long)bytes);
+    if (ext_ws_bytes < L.bytes)
+      return set_error(ctx, QTNH_EINVAL, "contract: workspace of %llu bytes, need %llu",
+                       (unsigned long long)ext_ws_bytes, (unsigned long long)L.bytes);
     ws = (char *)ext_ws;
-  } else if (bytes) {
+  } else if (L.bytes) {
     void *w;
-    qtnh_status st = workspace(ctx, bytes, &w);
+    qtnh_status st = workspace(ctx, L.bytes, &w);
     if (st) return st;
     ws = (char *)w;
   }
-  const c128 *Ap = A, *Bp = B;
-  c128 *Cn = C;
-  uint64_t off = 0;
+  const c128 *Ap = L.idA ? A : (const c128 *)(ws + L.offA);
+  const c128 *Bp = L.idB ? B : (const c128 *)(ws + L.offB);
+  c128 *Cn = L.idC ? C : (c128 *)(ws + L.offC);
   if (!L.idA) {
-    c128 *t = (c128 *)(ws + off);
-    off += pad_c128(P.M * P.K);
-    qtnh_status st = launch_permute(ctx, A, t, rA, dA, L.pA.data());
+    qtnh_status st = launch_permute(ctx, A, (c128 *)Ap, rA, dA, L.pA.data());
     if (st) return st;
-    Ap = t;
   }
   if (!L.idB) {
-    c128 *t = (c128 *)(ws + off);
-    off += pad_c128(P.K * P.N);
-    qtnh_status st = launch_permute(ctx, B, t, rB, dB, L.pB.data());
+    qtnh_status st = launch_permute(ctx, B, (c128 *)Bp, rB, dB, L.pB.data());
     if (st) return st;
-    Bp = t;
-  }
-  if (!L.idC) {
-    Cn = (c128 *)(ws + off);
-    off += pad_c128(P.nC);
   }
   qtnh_status st;
   if (L.splitk) {
-    st = launch_zgemm_splitk(ctx, Ap, Bp, Cn, P.M, P.N, P.K, L.kchunks, (c128 *)(ws + off));
+    st = launch_zgemm_splitk(ctx, Ap, Bp, Cn, P.M, P.N, P.K, L.kchunks, (c128 *)(ws + L.offS));
     if (err_out) *err_out = 0.0;
   } else if (L.split) {
-    st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + off, L.sbytes, err_out);
+    st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + L.offS, L.sbytes, err_out);
   } else {
     st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N,
-                          L.zchunks > 1 ? (c128 *)(ws + off) : nullptr);
+                          L.zchunks > 1 ? (c128 *)(ws + L.offS) : nullptr);
     if (err_out) *err_out = 0.0;
   }
   if (st) return st;
   if (!L.idC) {
-    // Natural result dims: open1 dims then open2 dims.
-    std::vector<uint64_t> dn;
-    for (int i : P.open1) dn.push_back(dA[i]);
-    for (int i : P.open2) dn.push_back(dB[i]);
-    st = launch_permute(ctx, Cn, C, P.rC, dn.data(), L.pC.data());
+    st = launch_permute(ctx, Cn, C, P.rC, L.dn.data(), L.pC.data());
     if (st) return st;
   }
   return QTNH_OK;

@@ -109,6 +109,24 @@ struct ContractRoute {
 };
 ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
                              const uint16_t *repl1, const uint16_t *repl2, int precision);
+// The GEMM route of launch_contract for these arguments (false: the direct
+// route): A' = A permuted by pA to [open1, closed]; B' = B permuted by pB to
+// [closed, open2] (FP64 GEMM) or [open2, closed] (split-K, split precision);
+// the natural result [open1, open2] (dims dn) permuted by pC to the placement.
+// An operand already in its layout (idA / idB / idC) is used in place.
+// Workspace (bytes from its start): A' at offA, B' at offB, natural C at
+// offC, then split scratch, split-K partials or depth-chunk partials at offS.
+struct GemmRoute {
+  bool splitk = false, split = false;
+  bool idA = true, idB = true, idC = true;
+  std::vector<uint16_t> pA, pB, pC;
+  std::vector<uint64_t> dn;
+  uint64_t M = 0, N = 0, K = 0, nC = 0;
+  uint64_t kchunks = 0, zchunks = 1, sbytes = 0, bytes = 0;
+  uint64_t offA = 0, offB = 0, offC = 0, offS = 0;
+};
+bool gemm_route(int rA, const uint64_t *dA, int rB, const uint64_t *dB, int nw, const uint16_t *wires,
+                const uint16_t *repl1, const uint16_t *repl2, int precision, GemmRoute *out);
 // Direct kernels only (reference summation order, no workspace use).
 qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_t *dA, const c128 *B, int rB,
                                    const uint64_t *dB, int nw, const uint16_t *wires, const uint16_t *repl1,
@@ -138,6 +156,57 @@ uint64_t zgemm_chunks(uint64_t M, uint64_t N, uint64_t K);
 // in chunk order.
 qtnh_status launch_zgemm_splitk(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
                                 uint64_t K, uint64_t chunks, c128 *partials);
+
+// ---- grouped launches: many independent problems in one launch, each a
+// contiguous range of blocks starting at `first` (descriptors sorted by it).
+// One GEMM of launch_zgemm_grouped: C[MxN] = A[MxK] * B[KxN], all contiguous
+// row-major; `tiles` output tiles (64 x block width) times the depth chunks of
+// zgemm_chunk_shape (chunk c sums k in [c kc, (c + 1) kc) into C + c cstride).
+struct ZGroupDesc {
+  const c128 *A;
+  const c128 *B;
+  c128 *C;
+  uint64_t M, N, K, kc, cstride;
+  uint64_t first, tiles;
+};
+// Depth-chunk sums: C[i] = sum over c of part[c MN + i] (256 outputs per block).
+struct ZSumDesc {
+  const c128 *part;
+  c128 *C;
+  uint64_t MN, chunks, first;
+};
+// Split-K contractions: partial blocks [first, first + MN S), final blocks
+// [first2, first2 + ceil(MN / 256)).
+struct SkDesc {
+  const c128 *A;
+  const c128 *Bt;
+  c128 *part;
+  c128 *C;
+  uint64_t N, K, kc, S, MN;
+  uint64_t first, first2;
+};
+// A bit permutation (every dim a power of two): element j of dst is element
+// sum_p bit_p(j) << sbit[p] of src; 1024 elements per block.
+struct PermDesc {
+  const c128 *src;
+  c128 *dst;
+  uint64_t n, first;
+  int nb;
+  int8_t sbit[64];
+};
+qtnh_status launch_zgemm_grouped(qtnh_ctx_t ctx, int bn, const ZGroupDesc *d, int nd, uint64_t total_blocks);
+qtnh_status launch_zgemm_grouped_sum(qtnh_ctx_t ctx, const ZSumDesc *d, int nd, uint64_t total_blocks);
+qtnh_status launch_splitk_grouped(qtnh_ctx_t ctx, const SkDesc *d, int nd, uint64_t total_partial,
+                                  uint64_t total_final);
+qtnh_status launch_permute_grouped(qtnh_ctx_t ctx, const PermDesc *d, int nd, uint64_t total_blocks);
+// Block width (64 or 128) and depth chunks (length on BK steps, count) of launch_zgemm_f64.
+int zgemm_block_n(uint64_t M, uint64_t N);
+void zgemm_chunk_shape(uint64_t M, uint64_t N, uint64_t K, uint64_t *kc, uint64_t *nchunks);
+// Terms per chunk of the split-K route (a multiple of its workgroup size).
+uint64_t splitk_chunk_len(uint64_t K, uint64_t chunks);
+// Source bit of every result bit of a permutation of power-of-two dims
+// (false when a dim is not a power of two).
+bool permute_source_bits(int rank, const uint64_t *dims, const uint16_t *ptup, int *nb, int8_t *sbit);
 
 // Split-precision complex GEMM C[MxN] = A[MxK] * Bt[NxK]^T (all contiguous,
 // row-major) on INT8 matrix cores; scratch of split_scratch_bytes(M, N, K).

@@ -153,6 +153,8 @@ struct qtnh_net_s {
     bool direct = true;
     bool binary = true;
     uint64_t ws = 0;
+    uint64_t ws64 = 0;       // workspace of the step's FP64 route
+    bool groupable = false;  // FP64 GEMM-route step small enough for the batch's grouped launches
     uint64_t M = 0, N = 0;  // GEMM shape of the route (open A x open B)
     double flops = 0, bytes = 0;
   };
@@ -184,6 +186,14 @@ GDesc gdesc_template(const qtnh_net_s &net, size_t s);
 // 256 CUs; narrower GEMMs (few tiles, long sums) are faster in FP64, where
 // depth chunks fill the chip, and exact.
 constexpr uint64_t kSplitMinTiles = 256;
+
+// Batches run the GEMM-route steps of one step index across networks in
+// grouped launches when each is small (operands and result of at most 2^22
+// elements, FP64 workspace of at most 64 MiB; larger ones fill the GPU alone)
+// and the step index's grouped workspaces stay within 1 GiB.
+constexpr uint64_t kGroupMaxElems = 1ull << 22;
+constexpr uint64_t kGroupMaxWs = 64ull << 20;
+constexpr uint64_t kGroupWsTotal = 1ull << 30;
 int step_precision(const qtnh_net_s::Step &st, int precision) {
   if (precision != QTNH_PREC_SPLIT) return QTNH_PREC_FP64;
   return ((st.M + 127) / 128) * ((2 * st.N + 127) / 128) >= kSplitMinTiles ? QTNH_PREC_SPLIT : QTNH_PREC_FP64;
@@ -275,6 +285,9 @@ void add_step(qtnh_net_s &net, qtnh_net_s::Step st, qtnh_net_s::Node r, Arena &a
   st.N = R.N;
   st.ws = step_precision(st, net.precision) == QTNH_PREC_SPLIT ? step_route(net, st, QTNH_PREC_SPLIT).ws_bytes
                                                                 : R.ws_bytes;
+  st.ws64 = R.ws_bytes;
+  st.groupable = !st.direct && st.binary && A.size <= kGroupMaxElems && B.size <= kGroupMaxElems &&
+                 r.size <= kGroupMaxElems && st.ws64 <= kGroupMaxWs;
   net.ws_bytes = std::max(net.ws_bytes, st.ws);
   r.offset = arena.alloc(r.size);
   net.max_rank = std::max(net.max_rank, (int)r.dims.size());
@@ -555,9 +568,49 @@ namespace {
 // Descriptors per upload-and-launch range of an eager batch pass.
 constexpr size_t kDescChunk = 2048;
 
+// Whether step s of network n runs in its step index's grouped launches
+// (given the running workspace of the step index's earlier grouped steps).
+bool grouped_step(const qtnh_net_s &n, int prec, size_t s, uint64_t sum) {
+  if (s >= n.steps.size()) return false;
+  const auto &st = n.steps[s];
+  return st.groupable && step_precision(st, prec) == QTNH_PREC_FP64 && sum + st.ws64 <= kGroupWsTotal;
+}
+
+// Device bytes of a wave (a consecutive run of networks), network by network:
+// leaves, arenas and results, plus a workspace for the larger of one
+// individual step and one step index's grouped steps.
+struct WaveBytes {
+  uint64_t elems = 0, res = 0, ws = 0, gmax = 0;
+  std::vector<uint64_t> gsum;
+  void add(const qtnh_net_s &n) {
+    elems += block_elems(n);
+    res += n.nodes[n.result].size;
+    ws = std::max(ws, n.ws_bytes);
+    if (gsum.size() < n.steps.size()) gsum.resize(n.steps.size(), 0);
+    for (size_t s = 0; s < n.steps.size(); ++s)
+      if (grouped_step(n, n.precision, s, gsum[s])) {
+        gsum[s] += n.steps[s].ws64;
+        gmax = std::max(gmax, gsum[s]);
+      }
+  }
+  uint64_t bytes() const { return (elems + round16(res)) * sizeof(c128) + std::max(ws, gmax); }
+};
+
+// Descriptor ranges of one step index in the grouped lists.
+struct GRange {
+  uint32_t b = 0, e = 0;
+  uint64_t blocks = 0;
+};
+
 // One batch of networks placed in one device block:
 //   [leaves of every net | arenas | packed results | GEMM workspace]
 // plus descriptor memory (grouped-step descriptors, result gather table).
+// Step index s runs as: one grouped launch of its small direct steps; its
+// small FP64 GEMM-route steps as grouped launches (operand permutes, GEMM
+// tiles of each block width, depth-chunk sums, split-K, result permutes), each
+// with a workspace slice of its own; then its other steps one at a time.
+// Grouped and single launches do the same arithmetic in the same order, so
+// results equal the one-at-a-time executor's bit for bit.
 struct Wave {
   std::vector<const qtnh_net_s *> nets;
   std::vector<int> precs;  // each network's precision when the batch was laid out
@@ -574,15 +627,23 @@ struct Wave {
   size_t steps = 0;
   GDesc *ddesc = nullptr;
   Gather *dgath = nullptr;
+  // Grouped GEMM-route steps: host lists (all step indices) and ranges per step index.
+  std::vector<std::vector<char>> grouped;  // [network][step]
+  std::vector<PermDesc> pin, pout;
+  std::vector<ZGroupDesc> zg[2];  // block width 64, 128
+  std::vector<ZSumDesc> zsum;
+  std::vector<SkDesc> sk;
+  std::vector<GRange> r_pin, r_pout, r_zg[2], r_zsum, r_sk;
+  std::vector<uint64_t> r_sk_final;  // split-K final blocks per step index
+  PermDesc *d_pin = nullptr, *d_pout = nullptr;
+  ZGroupDesc *d_zg[2] = {nullptr, nullptr};
+  ZSumDesc *d_zsum = nullptr;
+  SkDesc *d_sk = nullptr;
 
   static uint64_t device_bytes(const qtnh_net_t *nets, int count) {
-    uint64_t elems = 0, res = 0, ws = 0;
-    for (int i = 0; i < count; ++i) {
-      elems += block_elems(*nets[i]);
-      res += nets[i]->nodes[nets[i]->result].size;
-      ws = std::max(ws, nets[i]->ws_bytes);
-    }
-    return (elems + round16(res)) * sizeof(c128) + ws;
+    WaveBytes w;
+    for (int i = 0; i < count; ++i) w.add(*nets[i]);
+    return w.bytes();
   }
 
   // Placement of every network in `mem`, the leaf image, the descriptor count
@@ -592,13 +653,15 @@ struct Wave {
     nets.assign(ns, ns + count);
     for (auto *n : nets) precs.push_back(n->precision);
     uint64_t elems = 0;
+    WaveBytes wb;
     for (auto *n : nets) {
       elems += block_elems(*n);
       res_elems += n->nodes[n->result].size;
       leaf_elems += round16(n->leaf_elems);
-      ws_bytes = std::max(ws_bytes, n->ws_bytes);
       steps = std::max(steps, n->steps.size());
+      wb.add(*n);
     }
+    ws_bytes = std::max(wb.ws, wb.gmax);
     base = (c128 *)mem;
     res = base + elems;
     gws = (char *)(res + round16(res_elems));
@@ -620,11 +683,123 @@ struct Wave {
       fill_descs(0, steps, descs.data());
       hdesc = descs.data();
     }
+    layout_grouped();
     uint64_t dst = 0;
     for (size_t i = 0; i < nets.size(); ++i) {
       const auto &n = *nets[i];
       gathers.push_back({(const double2_t *)node_ptr(n, blocks[i], n.result), n.nodes[n.result].size, dst});
       dst += n.nodes[n.result].size;
+    }
+  }
+
+  // Descriptors of the grouped GEMM-route steps of every step index.
+  void layout_grouped() {
+    grouped.assign(nets.size(), std::vector<char>());
+    for (size_t i = 0; i < nets.size(); ++i) grouped[i].assign(nets[i]->steps.size(), 0);
+    r_pin.assign(steps, GRange());
+    r_pout.assign(steps, GRange());
+    r_zg[0].assign(steps, GRange());
+    r_zg[1].assign(steps, GRange());
+    r_zsum.assign(steps, GRange());
+    r_sk.assign(steps, GRange());
+    r_sk_final.assign(steps, 0);
+    auto open_range = [](GRange &r, size_t size) { r.b = r.e = (uint32_t)size; };
+    for (size_t s = 0; s < steps; ++s) {
+      open_range(r_pin[s], pin.size());
+      open_range(r_pout[s], pout.size());
+      open_range(r_zg[0][s], zg[0].size());
+      open_range(r_zg[1][s], zg[1].size());
+      open_range(r_zsum[s], zsum.size());
+      open_range(r_sk[s], sk.size());
+      uint64_t sum = 0;
+      for (size_t i = 0; i < nets.size(); ++i) {
+        const auto &n = *nets[i];
+        if (!grouped_step(n, precs[i], s, sum)) continue;
+        const auto &st = n.steps[s];
+        // The slice is taken before the checks below, as WaveBytes counts it.
+        char *w = gws + sum;
+        sum += st.ws64;
+        const auto &A = n.nodes[st.a], &B = n.nodes[st.b], &Cn = n.nodes[st.c];
+        std::vector<uint16_t> r1, r2;
+        step_repls(n, st, r1, r2);
+        GemmRoute R;
+        if (!gemm_route((int)A.dims.size(), A.dims.data(), (int)B.dims.size(), B.dims.data(),
+                        (int)st.wires.size() / 2, st.wires.data(), r1.data(), r2.data(), QTNH_PREC_FP64, &R) ||
+            R.split)
+          continue;
+        PermDesc pa, pb, pc;
+        if ((!R.idA && !permute_source_bits((int)A.dims.size(), A.dims.data(), R.pA.data(), &pa.nb, pa.sbit)) ||
+            (!R.idB && !permute_source_bits((int)B.dims.size(), B.dims.data(), R.pB.data(), &pb.nb, pb.sbit)) ||
+            (!R.idC && !permute_source_bits((int)Cn.dims.size(), R.dn.data(), R.pC.data(), &pc.nb, pc.sbit)))
+          continue;
+        grouped[i][s] = 1;
+        const c128 *Ap = node_ptr(n, blocks[i], st.a), *Bp = node_ptr(n, blocks[i], st.b);
+        c128 *Cp = node_ptr(n, blocks[i], st.c);
+        const c128 *Aw = R.idA ? Ap : (const c128 *)(w + R.offA);
+        const c128 *Bw = R.idB ? Bp : (const c128 *)(w + R.offB);
+        c128 *Cw = R.idC ? Cp : (c128 *)(w + R.offC);
+        auto add_perm = [](std::vector<PermDesc> &v, GRange &r, PermDesc d, const c128 *src, c128 *dst,
+                           uint64_t count) {
+          d.src = src;
+          d.dst = dst;
+          d.n = count;
+          d.first = r.blocks;
+          r.blocks += (count + 1023) / 1024;
+          v.push_back(d);
+          r.e = (uint32_t)v.size();
+        };
+        if (!R.idA) add_perm(pin, r_pin[s], pa, Ap, (c128 *)Aw, R.M * R.K);
+        if (!R.idB) add_perm(pin, r_pin[s], pb, Bp, (c128 *)Bw, R.K * R.N);
+        if (R.splitk) {
+          SkDesc d;
+          d.A = Aw;
+          d.Bt = Bw;
+          d.part = (c128 *)(w + R.offS);
+          d.C = Cw;
+          d.N = R.N;
+          d.K = R.K;
+          d.S = R.kchunks;
+          d.kc = splitk_chunk_len(R.K, R.kchunks);
+          d.MN = R.M * R.N;
+          d.first = r_sk[s].blocks;
+          d.first2 = r_sk_final[s];
+          r_sk[s].blocks += d.MN * d.S;
+          r_sk_final[s] += (d.MN + 255) / 256;
+          sk.push_back(d);
+          r_sk[s].e = (uint32_t)sk.size();
+        } else {
+          const int bi = zgemm_block_n(R.M, R.N) == 128 ? 1 : 0;
+          const uint64_t bn = bi ? 128 : 64;
+          uint64_t kc = 0, nch = 1;
+          zgemm_chunk_shape(R.M, R.N, R.K, &kc, &nch);
+          ZGroupDesc d;
+          d.A = Aw;
+          d.B = Bw;
+          d.C = nch > 1 ? (c128 *)(w + R.offS) : Cw;
+          d.M = R.M;
+          d.N = R.N;
+          d.K = R.K;
+          d.kc = kc;
+          d.cstride = nch > 1 ? R.M * R.N : 0;
+          d.tiles = ((R.N + bn - 1) / bn) * ((R.M + 63) / 64);
+          d.first = r_zg[bi][s].blocks;
+          r_zg[bi][s].blocks += d.tiles * nch;
+          zg[bi].push_back(d);
+          r_zg[bi][s].e = (uint32_t)zg[bi].size();
+          if (nch > 1) {
+            ZSumDesc z;
+            z.part = (const c128 *)(w + R.offS);
+            z.C = Cw;
+            z.MN = R.M * R.N;
+            z.chunks = nch;
+            z.first = r_zsum[s].blocks;
+            r_zsum[s].blocks += (z.MN + 255) / 256;
+            zsum.push_back(z);
+            r_zsum[s].e = (uint32_t)zsum.size();
+          }
+        }
+        if (!R.idC) add_perm(pout, r_pout[s], pc, Cw, Cp, R.nC);
+      }
     }
   }
 
@@ -643,17 +818,36 @@ struct Wave {
     }
   }
 
+  static uint64_t pad256(uint64_t b) { return (b + 255) & ~255ull; }
+
   uint64_t desc_bytes() const {
-    return ((ndesc * sizeof(GDesc) + 255) & ~255ull) + gathers.size() * sizeof(Gather) + 256;
+    return pad256(ndesc * sizeof(GDesc)) + pad256(gathers.size() * sizeof(Gather)) +
+           pad256(pin.size() * sizeof(PermDesc)) + pad256(pout.size() * sizeof(PermDesc)) +
+           pad256(zg[0].size() * sizeof(ZGroupDesc)) + pad256(zg[1].size() * sizeof(ZGroupDesc)) +
+           pad256(zsum.size() * sizeof(ZSumDesc)) + pad256(sk.size() * sizeof(SkDesc)) + 256;
   }
 
   void place_descriptors(void *mem) {
-    ddesc = (GDesc *)mem;
-    dgath = (Gather *)((char *)mem + ((ndesc * sizeof(GDesc) + 255) & ~255ull));
+    char *p = (char *)mem;
+    ddesc = (GDesc *)p;
+    p += pad256(ndesc * sizeof(GDesc));
+    dgath = (Gather *)p;
+    p += pad256(gathers.size() * sizeof(Gather));
+    d_pin = (PermDesc *)p;
+    p += pad256(pin.size() * sizeof(PermDesc));
+    d_pout = (PermDesc *)p;
+    p += pad256(pout.size() * sizeof(PermDesc));
+    d_zg[0] = (ZGroupDesc *)p;
+    p += pad256(zg[0].size() * sizeof(ZGroupDesc));
+    d_zg[1] = (ZGroupDesc *)p;
+    p += pad256(zg[1].size() * sizeof(ZGroupDesc));
+    d_zsum = (ZSumDesc *)p;
+    p += pad256(zsum.size() * sizeof(ZSumDesc));
+    d_sk = (SkDesc *)p;
   }
 
-  // Leaves, the gather table and (when built in full) the descriptors to the
-  // device (stream-ordered).
+  // Leaves, the gather table, the grouped GEMM-route descriptors and (when
+  // built in full) the direct-step descriptors to the device (stream-ordered).
   qtnh_status upload(qtnh_ctx_t ctx) {
     qtnh_status st = check_hip(
         ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
@@ -666,6 +860,35 @@ struct Wave {
       st = check_hip(ctx, hipMemcpyAsync(dgath, gathers.data(), gathers.size() * sizeof(Gather), hipMemcpyHostToDevice,
                                          ctx->stream),
                      "batch gather upload");
+    auto put = [&](void *dst, const void *src, size_t bytes) {
+      if (!st && bytes)
+        st = check_hip(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream),
+                       "batch grouped descriptor upload");
+    };
+    put(d_pin, pin.data(), pin.size() * sizeof(PermDesc));
+    put(d_pout, pout.data(), pout.size() * sizeof(PermDesc));
+    put(d_zg[0], zg[0].data(), zg[0].size() * sizeof(ZGroupDesc));
+    put(d_zg[1], zg[1].data(), zg[1].size() * sizeof(ZGroupDesc));
+    put(d_zsum, zsum.data(), zsum.size() * sizeof(ZSumDesc));
+    put(d_sk, sk.data(), sk.size() * sizeof(SkDesc));
+    return st;
+  }
+
+  // The grouped GEMM-route launches of step index s.
+  qtnh_status enqueue_grouped(qtnh_ctx_t ctx, size_t s) {
+    qtnh_status st = QTNH_OK;
+    const GRange &a = r_pin[s];
+    if (a.e > a.b) st = launch_permute_grouped(ctx, d_pin + a.b, (int)(a.e - a.b), a.blocks);
+    for (int bi = 0; bi < 2 && !st; ++bi) {
+      const GRange &z = r_zg[bi][s];
+      if (z.e > z.b) st = launch_zgemm_grouped(ctx, bi ? 128 : 64, d_zg[bi] + z.b, (int)(z.e - z.b), z.blocks);
+    }
+    const GRange &c = r_zsum[s];
+    if (!st && c.e > c.b) st = launch_zgemm_grouped_sum(ctx, d_zsum + c.b, (int)(c.e - c.b), c.blocks);
+    const GRange &k = r_sk[s];
+    if (!st && k.e > k.b) st = launch_splitk_grouped(ctx, d_sk + k.b, (int)(k.e - k.b), k.blocks, r_sk_final[s]);
+    const GRange &o = r_pout[s];
+    if (!st && o.e > o.b) st = launch_permute_grouped(ctx, d_pout + o.b, (int)(o.e - o.b), o.blocks);
     return st;
   }
 
@@ -689,10 +912,12 @@ struct Wave {
         qtnh_status st = check_hip(ctx, hipGetLastError(), "grouped contraction launch");
         if (st) return st;
       }
+      qtnh_status st = enqueue_grouped(ctx, s);
+      if (st) return st;
       for (size_t i = 0; i < nets.size(); ++i) {
         const auto &n = *nets[i];
-        if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary)) continue;
-        qtnh_status st = run_step(ctx, n, blocks[i], s, gws, ws_bytes, precs[i]);
+        if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary) || grouped[i][s]) continue;
+        st = run_step(ctx, n, blocks[i], s, gws, ws_bytes, precs[i]);
         if (st) return st;
       }
     }
@@ -1009,9 +1234,13 @@ qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets
   for (int w0 = 0; w0 < count;) {
     uint64_t bytes = 0;
     int w1 = w0;
+    WaveBytes wb;
     while (w1 < count) {
-      const uint64_t b2 = Wave::device_bytes(nets + w0, w1 + 1 - w0);
+      WaveBytes next = wb;
+      next.add(*nets[w1]);
+      const uint64_t b2 = next.bytes();
       if (w1 > w0 && b2 > budget_bytes) break;
+      wb = std::move(next);
       bytes = b2;
       ++w1;
     }
@@ -1127,6 +1356,27 @@ qtnh_status qtnh_net_batch_run(qtnh_ctx_t ctx, qtnh_net_batch_t b, double *const
     st = b->wave.enqueue(ctx);
   if (st) return st;
   return b->wave.fetch(ctx, outs);
+}
+
+qtnh_status qtnh_net_batch_stats(qtnh_net_batch_t b, uint64_t *grouped_direct, uint64_t *grouped_gemm,
+                                 uint64_t *single) {
+  if (!b) return QTNH_EINVAL;
+  uint64_t gd = 0, gg = 0, sg = 0;
+  const Wave &w = b->wave;
+  for (size_t i = 0; i < w.nets.size(); ++i)
+    for (size_t s = 0; s < w.nets[i]->steps.size(); ++s) {
+      const auto &st = w.nets[i]->steps[s];
+      if (st.direct && st.binary)
+        ++gd;
+      else if (w.grouped[i][s])
+        ++gg;
+      else
+        ++sg;
+    }
+  if (grouped_direct) *grouped_direct = gd;
+  if (grouped_gemm) *grouped_gemm = gg;
+  if (single) *single = sg;
+  return QTNH_OK;
 }
 
 qtnh_status qtnh_net_batch_destroy(qtnh_net_batch_t b) {

@@ -556,4 +556,62 @@ qtnh_status launch_swap(qtnh_ctx_t ctx, c128 *buf, int rank, const uint64_t *dim
   return check_hip(ctx, hipGetLastError(), "swap_general launch");
 }
 
+
+namespace {
+// Many bit permutations in one launch (the small operand permutes of a
+// network batch's steps): block b of descriptor D moves dst elements
+// [1024 b, 1024 b + 1024) of D, thread t the elements 1024 b + t + 256 i,
+// so each store instruction writes 4 KiB of consecutive elements.  The source
+// offset of an element splits over its disjoint bits: that of the block's
+// base (bits 10 and up) OR that of t + 256 i (bits 0..9).
+__global__ void __launch_bounds__(256) permute_grouped_kernel(const PermDesc *__restrict__ d, int nd,
+                                                              uint64_t total) {
+  for (uint64_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (d[mid].first <= blk)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const PermDesc &D = d[lo];
+    const uint64_t base = (blk - D.first) << 10;
+    uint64_t sbase = 0;
+    for (int p = 10; p < D.nb; ++p)
+      if ((base >> p) & 1ull) sbase |= 1ull << D.sbit[p];
+    const int lowbits = D.nb < 10 ? D.nb : 10;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t low = threadIdx.x + 256u * i;
+      const uint64_t j = base + low;
+      if (j >= D.n) continue;
+      uint64_t s = sbase;
+      for (int p = 0; p < lowbits; ++p)
+        if ((low >> p) & 1u) s |= 1ull << D.sbit[p];
+      D.dst[j] = D.src[s];
+    }
+  }
+}
+}  // namespace
+
+bool permute_source_bits(int rank, const uint64_t *dims, const uint16_t *ptup, int *nb, int8_t *sbit) {
+  uint64_t nd[kMaxRank];
+  for (int i = 0; i < rank; ++i) nd[ptup[i]] = dims[i];
+  const BitLayout in = bit_layout(rank, dims);
+  const BitLayout out = bit_layout(rank, nd);
+  if (!in.binary || !out.binary || in.nbits > 64) return false;
+  for (int i = 0; i < rank; ++i)
+    for (int t = 0; t < in.width[i]; ++t) sbit[out.shift[ptup[i]] + t] = (int8_t)(in.shift[i] + t);
+  *nb = in.nbits;
+  return true;
+}
+
+qtnh_status launch_permute_grouped(qtnh_ctx_t ctx, const PermDesc *d, int nd, uint64_t total) {
+  if (nd <= 0 || total == 0) return QTNH_OK;
+  const unsigned grid = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(permute_grouped_kernel, dim3(grid), dim3(256), 0, ctx->stream, d, nd, total);
+  return check_hip(ctx, hipGetLastError(), "grouped permute launch");
+}
+
 }  // namespace qtnh_hip

@@ -34,27 +34,26 @@ struct ZTile {
   static_assert(A_PER >= 1 && B_PER >= 1, "tile too small");
 };
 
-// blockIdx.y selects a chunk of the depth: chunk c sums k in [c kc, (c + 1) kc)
-// and writes its tile to C + c cstride (one chunk: the whole depth, into C).
+// One BM x BN output tile `tile` (row-major over the tile grid) of
+// C = A * B over depth chunk `chunk` (k in [chunk kc, (chunk + 1) kc)), written
+// to C + chunk cstride.  As / Bs: the workgroup's LDS staging buffers.
 template <int BM, int BN>
-__global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restrict__ A, const c128 *__restrict__ B,
-                                                             c128 *__restrict__ C, uint64_t M, uint64_t N, uint64_t K,
-                                                             uint64_t lda, uint64_t ldb, uint64_t ldc, uint64_t kc,
-                                                             uint64_t cstride) {
+__device__ __forceinline__ void zgemm_tile(const c128 *__restrict__ A, const c128 *__restrict__ B,
+                                           c128 *__restrict__ C, uint64_t M, uint64_t N, uint64_t K, uint64_t lda,
+                                           uint64_t ldb, uint64_t ldc, uint64_t kc, uint64_t cstride, uint64_t tile,
+                                           uint64_t chunk, double2_t *As, double2_t *Bs) {
   using Z = ZTile<BM, BN>;
-  __shared__ double2_t As[BM * Z::LDA];
-  __shared__ double2_t Bs[BK * Z::LDB];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = (wave >> 1) * 32;  // wave tile origin inside the block tile
   const int wn = (wave & 1) * Z::WN;
   const uint64_t ntn = (N + BN - 1) / BN;
-  const uint64_t m0 = ((uint64_t)blockIdx.x / ntn) * BM;
-  const uint64_t n0 = ((uint64_t)blockIdx.x % ntn) * BN;
-  const uint64_t kb = (uint64_t)blockIdx.y * kc;
+  const uint64_t m0 = (tile / ntn) * BM;
+  const uint64_t n0 = (tile % ntn) * BN;
+  const uint64_t kb = chunk * kc;
   const uint64_t ke = kb + kc < K ? kb + kc : K;
-  C += (uint64_t)blockIdx.y * cstride;
+  C += chunk * cstride;
 
   double4_t accR[2][Z::TN], accI[2][Z::TN];
 #pragma unroll
@@ -140,15 +139,74 @@ __global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restr
       }
 }
 
+// blockIdx.x: output tile; blockIdx.y: depth chunk (one chunk: the whole
+// depth, into C; several: partial tiles at C + chunk cstride).
+template <int BM, int BN>
+__global__ void __launch_bounds__(kThreads) zgemm_f64_kernel(const c128 *__restrict__ A, const c128 *__restrict__ B,
+                                                             c128 *__restrict__ C, uint64_t M, uint64_t N, uint64_t K,
+                                                             uint64_t lda, uint64_t ldb, uint64_t ldc, uint64_t kc,
+                                                             uint64_t cstride) {
+  using Z = ZTile<BM, BN>;
+  __shared__ double2_t As[BM * Z::LDA];
+  __shared__ double2_t Bs[BK * Z::LDB];
+  zgemm_tile<BM, BN>(A, B, C, M, N, K, lda, ldb, ldc, kc, cstride, blockIdx.x, blockIdx.y, As, Bs);
+}
+
+// Index of the descriptor whose block range holds block blk (descriptors
+// sorted by first block; the first one starts at 0).
+template <typename D>
+__device__ __forceinline__ int find_desc(const D *__restrict__ d, int nd, uint64_t blk) {
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].first <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Many GEMMs in one launch: each block of a grid-stride walk over every
+// GEMM's (chunk, tile) blocks computes one tile, exactly as zgemm_f64_kernel.
+template <int BN>
+__global__ void __launch_bounds__(kThreads) zgemm_grouped_kernel(const ZGroupDesc *__restrict__ d, int nd,
+                                                                 uint64_t total) {
+  using Z = ZTile<64, BN>;
+  __shared__ double2_t As[64 * Z::LDA];
+  __shared__ double2_t Bs[BK * Z::LDB];
+  for (uint64_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
+    const ZGroupDesc &D = d[find_desc(d, nd, blk)];
+    const uint64_t local = blk - D.first;
+    zgemm_tile<64, BN>(D.A, D.B, D.C, D.M, D.N, D.K, D.K, D.N, D.N, D.kc, D.cstride, local % D.tiles,
+                       local / D.tiles, As, Bs);
+  }
+}
+
+// Sums of depth-chunk partials of many GEMMs: C = sum over chunks of part[c], in chunk order.
+__global__ void zgemm_grouped_sum_kernel(const ZSumDesc *__restrict__ d, int nd, uint64_t total) {
+  for (uint64_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
+    const ZSumDesc &D = d[find_desc(d, nd, blk)];
+    const uint64_t i = (blk - D.first) * blockDim.x + threadIdx.x;
+    if (i >= D.MN) continue;
+    const double2_t *part = (const double2_t *)D.part;
+    double2_t s = part[i];
+    for (uint64_t c = 1; c < D.chunks; ++c) {
+      const double2_t v = part[c * D.MN + i];
+      s.x = s.x + v.x;
+      s.y = s.y + v.y;
+    }
+    ((double2_t *)D.C)[i] = s;
+  }
+}
+
 constexpr int kSkThreads = 256;
 
-// Block (output mn, chunk s): ordered per-thread sums, fixed tree in LDS.
-__global__ void __launch_bounds__(kSkThreads) splitk_partial_kernel(const double2_t *__restrict__ A,
-                                                                    const double2_t *__restrict__ Bt, uint64_t N,
-                                                                    uint64_t K, uint64_t kc, uint64_t S,
-                                                                    double2_t *__restrict__ part) {
-  __shared__ double2_t red[kSkThreads];
-  const uint64_t mn = blockIdx.x / S, s = blockIdx.x % S;
+// Partial sum s of output mn = m N + n: ordered per-thread sums over the
+// chunk's terms, then a fixed tree in LDS; red: the workgroup's LDS buffer.
+__device__ __forceinline__ void splitk_partial(const double2_t *__restrict__ A, const double2_t *__restrict__ Bt,
+                                               uint64_t N, uint64_t K, uint64_t kc, uint64_t mn, uint64_t s,
+                                               double2_t *__restrict__ out, double2_t *red) {
   const uint64_t m = mn / N, n = mn % N;
   const double2_t *a = A + m * K, *b = Bt + n * K;
   const uint64_t k1 = std::min(K, (s + 1) * kc);
@@ -169,28 +227,70 @@ __global__ void __launch_bounds__(kSkThreads) splitk_partial_kernel(const double
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) *out = red[0];
+  __syncthreads();  // red is reused by the workgroup's next partial
+}
+
+// Block (output mn, chunk s).
+__global__ void __launch_bounds__(kSkThreads) splitk_partial_kernel(const double2_t *__restrict__ A,
+                                                                    const double2_t *__restrict__ Bt, uint64_t N,
+                                                                    uint64_t K, uint64_t kc, uint64_t S,
+                                                                    double2_t *__restrict__ part) {
+  __shared__ double2_t red[kSkThreads];
+  splitk_partial(A, Bt, N, K, kc, blockIdx.x / S, blockIdx.x % S, part + blockIdx.x, red);
+}
+
+__device__ __forceinline__ double2_t splitk_sum(const double2_t *__restrict__ part, uint64_t mn, uint64_t S) {
+  double re = 0.0, im = 0.0;
+  for (uint64_t s = 0; s < S; ++s) {
+    const double2_t v = part[mn * S + s];
+    re = re + v.x;
+    im = im + v.y;
+  }
+  return double2_t{re, im};
 }
 
 __global__ void splitk_final_kernel(const double2_t *__restrict__ part, uint64_t MN, uint64_t S,
                                     double2_t *__restrict__ C) {
-  for (uint64_t mn = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; mn < MN; mn += (uint64_t)gridDim.x * blockDim.x) {
-    double re = 0.0, im = 0.0;
-    for (uint64_t s = 0; s < S; ++s) {
-      const double2_t v = part[mn * S + s];
-      re = re + v.x;
-      im = im + v.y;
-    }
-    C[mn] = double2_t{re, im};
+  for (uint64_t mn = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; mn < MN; mn += (uint64_t)gridDim.x * blockDim.x)
+    C[mn] = splitk_sum(part, mn, S);
+}
+
+// Split-K partials of many contractions: blocks [first, first + MN S) of a
+// descriptor are its (output, chunk) pairs.
+__global__ void __launch_bounds__(kSkThreads) splitk_grouped_partial_kernel(const SkDesc *__restrict__ d, int nd,
+                                                                            uint64_t total) {
+  __shared__ double2_t red[kSkThreads];
+  for (uint64_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
+    const SkDesc &D = d[find_desc(d, nd, blk)];
+    const uint64_t local = blk - D.first;
+    splitk_partial((const double2_t *)D.A, (const double2_t *)D.Bt, D.N, D.K, D.kc, local / D.S, local % D.S,
+                   (double2_t *)D.part + local, red);
   }
 }
 
+// Their final sums: element i of descriptor j in block (i / 256) + D.first2.
+__global__ void splitk_grouped_final_kernel(const SkDesc *__restrict__ d, int nd, uint64_t total) {
+  for (uint64_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
+    int lo = 0, hi = nd - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (d[mid].first2 <= blk)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const SkDesc &D = d[lo];
+    const uint64_t mn = (blk - D.first2) * blockDim.x + threadIdx.x;
+    if (mn < D.MN) ((double2_t *)D.C)[mn] = splitk_sum((const double2_t *)D.part, mn, D.S);
+  }
+}
 }  // namespace
 
 qtnh_status launch_zgemm_splitk(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
                                 uint64_t K, uint64_t chunks, c128 *partials) {
   const uint64_t MN = M * N;
-  const uint64_t kc = ((K + chunks - 1) / chunks + kSkThreads - 1) / kSkThreads * kSkThreads;
+  const uint64_t kc = splitk_chunk_len(K, chunks);
   if (MN * chunks > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "splitk: grid too large");
   hipLaunchKernelGGL(splitk_partial_kernel, dim3((unsigned)(MN * chunks)), dim3(kSkThreads), 0, ctx->stream,
                      (const double2_t *)A, (const double2_t *)Bt, N, K, kc, chunks, (double2_t *)partials);
@@ -269,6 +369,52 @@ qtnh_status launch_zgemm_f64(qtnh_ctx_t ctx, const c128 *A, const c128 *B, c128 
   hipLaunchKernelGGL(zgemm_chunk_sum_kernel, dim3(grid_for(M * N, 256, 65536)), dim3(256), 0, ctx->stream,
                      (const double2_t *)partials, (double2_t *)C, M, N, ldc, nchunks);
   return check_hip(ctx, hipGetLastError(), "zgemm chunk sum launch");
+}
+
+
+uint64_t splitk_chunk_len(uint64_t K, uint64_t chunks) {
+  return ((K + chunks - 1) / chunks + kSkThreads - 1) / kSkThreads * kSkThreads;
+}
+
+int zgemm_block_n(uint64_t M, uint64_t N) { return zgemm_bn(M, N); }
+
+// Depth chunks of launch_zgemm_f64 on BK steps: (chunk length, chunk count).
+void zgemm_chunk_shape(uint64_t M, uint64_t N, uint64_t K, uint64_t *kc_out, uint64_t *n_out) {
+  const uint64_t chunks = zgemm_chunks(M, N, K);
+  const uint64_t kc = chunks > 1 ? (K + chunks - 1) / chunks : K;
+  const uint64_t kcr = chunks > 1 ? (kc + BK - 1) / BK * BK : K;
+  *kc_out = kcr;
+  *n_out = chunks > 1 ? (K + kcr - 1) / kcr : 1;
+}
+
+qtnh_status launch_zgemm_grouped(qtnh_ctx_t ctx, int bn, const ZGroupDesc *d, int nd, uint64_t total) {
+  if (nd <= 0 || total == 0) return QTNH_OK;
+  const unsigned grid = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * 8);
+  if (bn == 128)
+    hipLaunchKernelGGL(zgemm_grouped_kernel<128>, dim3(grid), dim3(kThreads), 0, ctx->stream, d, nd, total);
+  else
+    hipLaunchKernelGGL(zgemm_grouped_kernel<64>, dim3(grid), dim3(kThreads), 0, ctx->stream, d, nd, total);
+  return check_hip(ctx, hipGetLastError(), "grouped zgemm launch");
+}
+
+qtnh_status launch_zgemm_grouped_sum(qtnh_ctx_t ctx, const ZSumDesc *d, int nd, uint64_t total) {
+  if (nd <= 0 || total == 0) return QTNH_OK;
+  const unsigned grid = (unsigned)std::min<uint64_t>(total, (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(zgemm_grouped_sum_kernel, dim3(grid), dim3(256), 0, ctx->stream, d, nd, total);
+  return check_hip(ctx, hipGetLastError(), "grouped chunk sum launch");
+}
+
+qtnh_status launch_splitk_grouped(qtnh_ctx_t ctx, const SkDesc *d, int nd, uint64_t total_partial,
+                                  uint64_t total_final) {
+  if (nd <= 0) return QTNH_OK;
+  unsigned grid = (unsigned)std::min<uint64_t>(total_partial, (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(splitk_grouped_partial_kernel, dim3(grid), dim3(kSkThreads), 0, ctx->stream, d, nd,
+                     total_partial);
+  qtnh_status st = check_hip(ctx, hipGetLastError(), "grouped splitk launch");
+  if (st) return st;
+  grid = (unsigned)std::min<uint64_t>(total_final, (uint64_t)ctx->num_cus * 16);
+  hipLaunchKernelGGL(splitk_grouped_final_kernel, dim3(grid), dim3(256), 0, ctx->stream, d, nd, total_final);
+  return check_hip(ctx, hipGetLastError(), "grouped splitk final launch");
 }
 
 }  // namespace qtnh_hip

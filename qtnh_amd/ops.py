@@ -544,6 +544,12 @@ class NetworkBatch:
         x, xp = _c128_host(data)
         check(lib().qtnh_net_batch_set_leaf(self._ctx, self._h, net, leaf, xp), self._ctx)
 
+    def stats(self):
+        """Steps by execution: grouped direct, grouped GEMM-route, one at a time."""
+        v = [ctypes.c_uint64(0) for _ in range(3)]
+        check(lib().qtnh_net_batch_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return {"grouped_direct": v[0].value, "grouped_gemm": v[1].value, "single": v[2].value}
+
     def run(self):
         outs = [np.zeros(max(_size(n.result_dims()), 1), dtype=np.complex128) for n in self.nets]
         ps = (ctypes.POINTER(ctypes.c_double) * len(outs))(*[o.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

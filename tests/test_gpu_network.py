@@ -187,3 +187,69 @@ def test_prepared_batch_keeps_its_precision():
         net.set_precision(_lib.PREC_FP64)
         eager.close()
         graph.close()
+
+
+def _two_leaf_net(m_bits, n_bits, k_bits, seed, a_wired_first=True, b_wired_first=False):
+    """A[m, k] x B[k, n] as a two-leaf network of binary axes with the wired
+    axes first or last in each leaf; returns (net, A, B, a_labels, b_labels)."""
+    rng = np.random.default_rng(seed)
+    opena = list(range(100, 100 + m_bits))
+    openb = list(range(200, 200 + n_bits))
+    wired = list(range(k_bits))
+    la = wired + opena if a_wired_first else opena + wired
+    lb = wired[::-1] + openb if b_wired_first else openb + wired[::-1]
+    A = rng.standard_normal(1 << (m_bits + k_bits)) + 1j * rng.standard_normal(1 << (m_bits + k_bits))
+    B = rng.standard_normal(1 << (n_bits + k_bits)) + 1j * rng.standard_normal(1 << (n_bits + k_bits))
+    net = ops.Network()
+    net.add([2] * len(la), la, A)
+    net.add([2] * len(lb), lb, B)
+    net.plan(0, 64)
+    return net, A, B, la, lb
+
+
+def _einsum(A, la, B, lb, out_labels):
+    letters = {}
+    def sym(v):
+        if v not in letters:
+            letters[v] = chr(ord("a") + len(letters)) if len(letters) < 26 else chr(ord("A") + len(letters) - 26)
+        return letters[v]
+    sa = "".join(sym(v) for v in la)
+    sb = "".join(sym(v) for v in lb)
+    so = "".join(sym(v) for v in out_labels)
+    return np.einsum("%s,%s->%s" % (sa, sb, so), A.reshape([2] * len(la)), B.reshape([2] * len(lb)),
+                     optimize=True).reshape(-1)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_grouped_gemm_route_steps_bitwise(use_graph):
+    """Mid-size GEMM-route steps of many networks run in grouped launches
+    (operand permutes, GEMM tiles with and without depth chunks, split-K):
+    bitwise equal to one network at a time, and equal to einsum within
+    1e-12 relative."""
+    nets, refs = [], []
+    shapes = [(9, 6, 11, True, False),   # 512 x 64 x 2048: depth chunks, both operands permuted
+              (8, 8, 6, True, False),    # 256 x 256 x 64: plain tiles
+              (4, 4, 14, False, True),   # 16 x 16 x 16384: split-K
+              (9, 6, 11, False, True),
+              (7, 9, 8, True, True)]
+    for i, (mb, nb, kb, af, bf) in enumerate(shapes * 2):
+        net, A, B, la, lb = _two_leaf_net(mb, nb, kb, 50 + i, af, bf)
+        out = [v for v in la if v >= 100] + [v for v in lb if v >= 100]
+        nets.append(net)
+        refs.append(_einsum(A, la, B, lb, out))
+    nets += mixed_networks()
+    solo = [n.run() for n in nets]
+    for s, r in zip(solo, refs):
+        assert np.linalg.norm(s - r) <= 1e-12 * np.linalg.norm(r)
+    batch, waves = ops.run_networks(nets)
+    assert waves == 1
+    prepared = ops.NetworkBatch(nets, use_graph=use_graph)
+    stats = prepared.stats()
+    assert stats["grouped_gemm"] >= len(shapes) * 2, stats
+    for rep in range(3):
+        got = prepared.run()
+        for s, b, g in zip(solo, batch, got):
+            assert same_bits(s, b)
+            assert same_bits(s, g), rep
+        torch.cuda.synchronize()
+    prepared.close()
