@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--calib-copy", action="store_true",
                    help="run one plain device copy of the state first (HBM counter calibration)")
     p.add_argument("--no-extra", action="store_true", help="skip the other configurations (C1, C3, C4, C5)")
-    p.add_argument("--only", default="", help="comma list of extra configurations to run (c2b,c1,c3c,c3g,c4,c5)")
+    p.add_argument("--only", default="", help="comma list of extra configurations to run (c2b,c2p,c1,c3c,c3g,c4,c5)")
     p.add_argument("--circuit-rank", type=int, default=30)
     p.add_argument("--circuit-depth", type=int, default=20)
     p.add_argument("--gemm-n", type=int, default=4096)
@@ -172,6 +172,74 @@ def bench_c2b(torch, ops, rank, seed):
     torch.cuda.empty_cache()
     return {"rank": rank, "gate_axes": gate_axes, "placement": "default (gate outputs last)", "ms": round(ms, 4),
             "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "GFLOP/s": round(8.0 * n * 4 / ms / 1e6, 1)}
+
+
+def bench_c2_placements(torch, ops, rank, seed):
+    """C2 kernels over fixed sets of placements instead of one seeded draw:
+    twelve permutations (eight seeded random ones, reversal, rotation,
+    lowest <-> highest axis, low 8 axes kept), gates on 1, 2 and 3 axes at
+    low, high and mixed positions, diagonals likewise.  GB/s = 32 N bytes per
+    launch over its time (HIP events, 5 launches after one untimed)."""
+    import numpy as np
+    n = 1 << rank
+    dims = [2] * rank
+    x = ops.fill_(ops.empty(n), seed)
+    y = ops.empty(n)
+    rng = random.Random(seed + 100)
+    perms = []
+    for _ in range(8):
+        p = list(range(rank))
+        rng.shuffle(p)
+        perms.append(("random", p))
+    perms.append(("reversal", list(range(rank))[::-1]))
+    perms.append(("rotation", [(i + 1) % rank for i in range(rank)]))
+    sw = list(range(rank))
+    sw[0], sw[rank - 1] = rank - 1, 0
+    perms.append(("lowest<->highest", sw))
+    hi = list(range(rank - 8))
+    rng.shuffle(hi)
+    perms.append(("low 8 kept", hi + list(range(rank - 8, rank))))
+    grng = np.random.default_rng(seed)
+
+    def gate(k):
+        m = 1 << k
+        return grng.standard_normal(m * m) + 1j * grng.standard_normal(m * m)
+
+    def phases(k):
+        return np.exp(2j * np.pi * grng.random(1 << k))
+
+    L = rank - 1
+    gates = [([L], gate(1)), ([0], gate(1)), ([rank // 2], gate(1)), ([L, L - 1], gate(2)), ([0, 1], gate(2)),
+             ([L, 0], gate(2)), ([rank // 2, rank // 2 + 1], gate(2)), ([L, L - 1, L - 2], gate(3)),
+             ([0, 1, 2], gate(3)), ([L, rank // 2, 0], gate(3))]
+    diags = [([L], phases(1)), ([0], phases(1)), ([L, L - 1], phases(2)), ([0, 1], phases(2)), ([L, 0], phases(2)),
+             ([L, rank // 2, 0], phases(3))]
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        return round(32.0 * n / ms / 1e6, 1)
+
+    out = {"rank": rank, "permute": [], "gate": [], "diag": []}
+    for name, p in perms:
+        out["permute"].append({"kind": name, "GB/s": timed(lambda: ops.permute(x, dims, p, out=y))})
+    for axes, g in gates:
+        out["gate"].append({"axes": axes, "GB/s": timed(lambda: ops.gate_apply_(x, dims, axes, g))})
+    for axes, d in diags:
+        out["diag"].append({"axes": axes, "GB/s": timed(lambda: ops.diag_apply_(x, dims, axes, d))})
+    for k in ("permute", "gate", "diag"):
+        fr = sorted(e["GB/s"] / HBM_PEAK_GBS for e in out[k])
+        out[k + "_frac"] = {"min": round(fr[0], 3), "median": round(float(np.median(fr)), 3), "max": round(fr[-1], 3)}
+    del x, y
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_c1(torch, ops, _lib):
@@ -484,6 +552,8 @@ def extra_configs(args):
     out = {}
     if want("c2b"):
         out["C2b_dense_gate_default_placement"] = bench_c2b(torch, ops, args.rank, args.seed)
+    if want("c2p"):
+        out["C2_placements"] = bench_c2_placements(torch, ops, args.rank, args.seed)
     if want("c1"):
         out["C1_qft12_latency"] = bench_c1(torch, ops, _lib)
     if want("c3c"):

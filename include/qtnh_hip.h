@@ -385,6 +385,10 @@ void qtnh_tune_permute_auto(int on);
 void qtnh_tune_zgemm(int bn);
 /* Number of 8-bit slices (1..8, default 5) of QTNH_PREC_SPLIT contractions. */
 void qtnh_tune_split(int slices);
+/* Block tile of the split-precision GEMM (up to 5 slices): 0 (default)
+ * 128 x 128, one workgroup per compute unit; 1 64 x 64, two per compute unit;
+ * 2 128 x 64; 3 64 x 64 with eight narrow waves.  Results do not depend on it. */
+void qtnh_tune_split_tile(int shape);
 /* Most workgroups (256..2^22, default 2^20) of the streaming gate kernel for
  * gates on high bits, which loops over the rest.  (The diagonal kernel and the
  * kernels for gates on the lowest 6 bits size their grids to the state.) */

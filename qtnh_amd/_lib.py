@@ -66,6 +66,7 @@ SIGNATURES = {
     "qtnh_tune_permute_auto": [_i],
     "qtnh_tune_zgemm": [_i],
     "qtnh_tune_split": [_i],
+    "qtnh_tune_split_tile": [_i],
     "qtnh_tune_circuit": [_i, _i],
     "qtnh_tune_stream": [_i],
     "qtnh_tune_circuit_pipeline": [_i],
@@ -107,7 +108,7 @@ SIGNATURES = {
 }
 _RESTYPE = {"qtnh_status_string": ctypes.c_char_p, "qtnh_ctx_last_error": ctypes.c_char_p,
             "qtnh_tune_permute": None, "qtnh_tune_permute_auto": None, "qtnh_tune_zgemm": None,
-            "qtnh_tune_split": None, "qtnh_tune_circuit": None,
+            "qtnh_tune_split": None, "qtnh_tune_split_tile": None, "qtnh_tune_circuit": None,
             "qtnh_tune_stream": None, "qtnh_tune_circuit_pipeline": None}
 
 _LIB = None

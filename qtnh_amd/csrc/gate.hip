@@ -125,6 +125,39 @@ __global__ void __launch_bounds__(kGateThreads) diag_bits_kernel(c128 *__restric
   }
 }
 
+// As diag_bits_kernel, for tables of more than 4 entries: the table is copied
+// into LDS once per workgroup and read from there per element (indexing the
+// argument block per lane would issue a memory load per element, which waits
+// behind the stream's own loads).  Same arithmetic as diag_bits_kernel.
+__global__ void __launch_bounds__(kGateThreads) diag_bits_lds_kernel(c128 *__restrict__ psi, DiagArgs args,
+                                                                     uint64_t n) {
+  constexpr int U = 4;
+  __shared__ double2_t tab[kMaxDiag];
+  const int K = 1 << args.kb;
+  for (int i = threadIdx.x; i < K; i += kGateThreads) tab[i] = double2_t{args.d[i].re, args.d[i].im};
+  __syncthreads();
+  double2_t *p2 = reinterpret_cast<double2_t *>(psi);
+  const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
+  for (uint64_t e0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; e0 < n; e0 += stride * U) {
+    double2_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t e = e0 + u * stride;
+      v[u] = e < n ? __builtin_nontemporal_load(p2 + e) : double2_t{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t e = e0 + u * stride;
+      if (e >= n) continue;
+      uint32_t c = 0;
+      for (int j = 0; j < args.kb; ++j) c = (c << 1) | (uint32_t)((e >> args.bitpos[j]) & 1ull);
+      const double2_t dc = tab[c];
+      const c128 r = cmul(c128{v[u].x, v[u].y}, c128{dc.x, dc.y});
+      __builtin_nontemporal_store(double2_t{r.re, r.im}, p2 + e);
+    }
+  }
+}
+
 struct DiagGenArgs {
   c128 d[kMaxDiag];
   uint64_t stride[kMaxRank];  // state stride of gate axis m
@@ -408,7 +441,10 @@ qtnh_status launch_diag(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *d
     // contiguous stretch of the state (81% of HBM peak at 2^28 and 2^30
     // elements, against 68-76% with four elements per thread 2^26+ apart).
     unsigned blocks = grid_for(n, kGateThreads, kMaxStreamBlocks);
-    hipLaunchKernelGGL(diag_bits_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, state, args, n);
+    if (K > 4)
+      hipLaunchKernelGGL(diag_bits_lds_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, state, args, n);
+    else
+      hipLaunchKernelGGL(diag_bits_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, state, args, n);
     return check_hip(ctx, hipGetLastError(), "diag launch");
   }
   DiagGenArgs args;

@@ -123,8 +123,8 @@ struct OzShape {
 
 // C (M x N2 real, leading dim ldc in doubles) (+)= recombined slice products.
 // As: S planes of M x K2 int8 (row-major, ld lda); Bt: S planes of N2 x K2 int8.
-template <int S, int WM, int WN, int WGM, int WGN>
-__global__ void __launch_bounds__(64 * WGM * WGN) ozaki_gemm_kernel(
+template <int S, int WM, int WN, int WGM, int WGN, int MINW>
+__global__ void __launch_bounds__(64 * WGM * WGN, MINW) ozaki_gemm_kernel(
     const int8_t *__restrict__ As, uint64_t a_plane, uint64_t lda, const int8_t *__restrict__ Bt, uint64_t b_plane,
     uint64_t ldb, const int *__restrict__ ea, const int *__restrict__ eb, double *__restrict__ C, uint64_t ldc,
     uint64_t M, uint64_t N2, uint64_t K2, unsigned tilesM, unsigned tilesN, int accumulate) {
@@ -357,12 +357,24 @@ __global__ void __launch_bounds__(512) ozaki_gemm_staged(
 }
 
 
-// Wide tiles while the stages fit two-deep; narrower ones for more slices.
-template <int S>
-struct OzPick {
-  static constexpr int WM = 64, WN = 32, WGM = 2, WGN = 4;
+// Tile shapes of ozaki_gemm_kernel (wave tile WM x WN, WGM x WGN waves, MINW
+// waves per SIMD).  0: 128 x 128 block tiles of 8 waves, one workgroup per
+// compute unit (both LDS stages fill the 160 KiB); 1: 64 x 64 of 4 waves, two
+// workgroups per compute unit (80 KiB each), so one's operand DMA overlaps the
+// other's matrix-core work; 2: 128 x 64 of 8 waves (120 KiB); 3: 64 x 64 of 8
+// waves with 32 x 16 wave tiles, two workgroups per compute unit.  Every shape
+// computes the same exact integer sums, so results do not depend on it.
+static int g_split_tile = 0;
+
+template <int S, int WM, int WN, int WGM, int WGN, int MINW>
+void launch_ozaki_shape(hipStream_t st, const int8_t *As, uint64_t ap, uint64_t lda, const int8_t *Bt, uint64_t bp,
+                        uint64_t ldb, const int *ea, const int *eb, double *C, uint64_t ldc, uint64_t M, uint64_t N2,
+                        uint64_t K2, int accumulate) {
   using Z = OzShape<S, WM, WN, WGM, WGN>;
-};
+  const unsigned tilesM = (unsigned)((M + Z::BM - 1) / Z::BM), tilesN = (unsigned)((N2 + Z::BN - 1) / Z::BN);
+  hipLaunchKernelGGL((ozaki_gemm_kernel<S, WM, WN, WGM, WGN, MINW>), dim3(tilesM * tilesN), dim3(Z::THREADS), 0, st,
+                     As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
+}
 
 template <int S>
 void launch_ozaki(hipStream_t st, const int8_t *As, uint64_t ap, uint64_t lda, const int8_t *Bt, uint64_t bp,
@@ -373,10 +385,20 @@ void launch_ozaki(hipStream_t st, const int8_t *As, uint64_t ap, uint64_t lda, c
     hipLaunchKernelGGL((ozaki_gemm_staged<S>), dim3(tilesM * tilesN), dim3(512), 0, st, As, ap, lda, Bt, bp, ldb, ea,
                        eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
   } else {
-  using P = OzPick<S>;
-  const unsigned tilesM = (unsigned)((M + P::Z::BM - 1) / P::Z::BM), tilesN = (unsigned)((N2 + P::Z::BN - 1) / P::Z::BN);
-  hipLaunchKernelGGL((ozaki_gemm_kernel<S, P::WM, P::WN, P::WGM, P::WGN>), dim3(tilesM * tilesN), dim3(P::Z::THREADS),
-                     0, st, As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, tilesM, tilesN, accumulate);
+    switch (g_split_tile) {
+      case 1:
+        launch_ozaki_shape<S, 32, 32, 2, 2, 2>(st, As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, accumulate);
+        break;
+      case 2:
+        launch_ozaki_shape<S, 64, 32, 2, 2, 1>(st, As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, accumulate);
+        break;
+      case 3:
+        launch_ozaki_shape<S, 32, 16, 2, 4, 4>(st, As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, accumulate);
+        break;
+      default:
+        launch_ozaki_shape<S, 64, 32, 2, 4, 1>(st, As, ap, lda, Bt, bp, ldb, ea, eb, C, ldc, M, N2, K2, accumulate);
+        break;
+    }
   }
 }
 
@@ -386,6 +408,8 @@ void launch_ozaki(hipStream_t st, const int8_t *As, uint64_t ap, uint64_t lda, c
 static int g_split_slices = 5;
 
 extern "C" void qtnh_tune_split(int slices) { g_split_slices = std::max(1, std::min(slices, kMaxSlices)); }
+
+extern "C" void qtnh_tune_split_tile(int shape) { g_split_tile = (shape >= 0 && shape <= 3) ? shape : 0; }
 
 int split_slices() { return g_split_slices; }
 

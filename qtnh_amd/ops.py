@@ -265,6 +265,11 @@ def tune_split(slices=5):
     lib().qtnh_tune_split(slices)
 
 
+def tune_split_tile(shape=0):
+    """Block tile of the split-precision GEMM (0..3, process-wide); results do not depend on it."""
+    lib().qtnh_tune_split_tile(int(shape))
+
+
 def tune_stream(max_blocks=1 << 20):
     """Most workgroups of the streaming kernel for gates on high bits (process-wide)."""
     lib().qtnh_tune_stream(max_blocks)

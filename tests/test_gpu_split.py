@@ -189,3 +189,25 @@ def test_split_small_problem_is_exact_fp64():
     cf, _, _ = ops.contract(dev(A), dA, dev(B), dB, [(1, 0)], precision=FP64)
     assert eta == 0.0
     assert np.array_equal(host(cs).view(np.uint64), host(cf).view(np.uint64))
+
+
+def test_split_block_tile_does_not_change_bits():
+    """Every block tile of the split GEMM sums the same exact integer products."""
+    rng = np.random.default_rng(77)
+    m, k, n = 384, 512, 320
+    A = rng.standard_normal(m * k) + 1j * rng.standard_normal(m * k)
+    B = rng.standard_normal(k * n) + 1j * rng.standard_normal(k * n)
+    da, db = [m, k], [k, n]
+    a = torch.from_numpy(A).cuda()
+    b = torch.from_numpy(B).cuda()
+    outs = []
+    try:
+        for shape in (0, 1, 2, 3):
+            ops.tune_split_tile(shape)
+            c, _, _ = ops.contract(a, da, b, db, [(1, 0)], precision=_lib.PREC_SPLIT)
+            torch.cuda.synchronize()
+            outs.append(c.cpu().numpy())
+    finally:
+        ops.tune_split_tile(0)
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint64), outs[0].view(np.uint64))
