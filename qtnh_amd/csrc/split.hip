@@ -31,6 +31,18 @@ typedef int int4_t __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kMaxSlices = 8;
 
+// Slice planes are stored in the order the GEMM kernels keep operands in LDS:
+// 1 KiB chunks of 16 rows x 64 bytes of depth, the K2 / 64 chunks of a row
+// group in depth order, row groups in row order.  Inside a chunk, 16-byte
+// slot 4 rr + (c ^ (rr / 2 % 4)) holds depth bytes 16 c .. 16 c + 15 of row
+// rr (the XOR spreads the fragment reads of 16 rows over distinct LDS banks).
+// A wave instruction then copies one whole chunk, 1 KiB contiguous, straight
+// into LDS.  Byte offset of real element (r, k) in a plane of depth K2:
+__device__ __forceinline__ uint64_t tiled_at(uint64_t r, uint64_t k, uint64_t K2) {
+  const uint64_t rr = r & 15, c = (k >> 4) & 3;
+  return ((((r >> 4) * (K2 >> 6)) + (k >> 6)) << 10) + ((rr * 4 + (c ^ ((rr >> 1) & 3))) << 4) + (k & 15);
+}
+
 // One block per complex row.  mode 0 (A): real row i = (re_0, im_0, re_1, ...).
 // mode 1 (B transposed, row j of B^T): real rows 2j = (re, -im, ...), 2j+1 = (im, re, ...).
 __global__ void __launch_bounds__(kThreads) split_rows_kernel(const c128 *__restrict__ X, uint64_t rows, uint64_t K,
@@ -84,10 +96,10 @@ __global__ void __launch_bounds__(kThreads) split_rows_kernel(const c128 *__rest
       if (p >= S) continue;
       int8_t *o = out + (uint64_t)p * plane;
       if (mode == 0) {
-        *reinterpret_cast<uint16_t *>(o + row * ldo + 2 * k) = pack(dr[p], di[p]);
+        *reinterpret_cast<uint16_t *>(o + tiled_at(row, 2 * k, ldo)) = pack(dr[p], di[p]);
       } else {
-        *reinterpret_cast<uint16_t *>(o + (2 * row) * ldo + 2 * k) = pack(dr[p], dn[p]);
-        *reinterpret_cast<uint16_t *>(o + (2 * row + 1) * ldo + 2 * k) = pack(di[p], dr[p]);
+        *reinterpret_cast<uint16_t *>(o + tiled_at(2 * row, 2 * k, ldo)) = pack(dr[p], dn[p]);
+        *reinterpret_cast<uint16_t *>(o + tiled_at(2 * row + 1, 2 * k, ldo)) = pack(di[p], dr[p]);
       }
     }
   }
@@ -122,7 +134,8 @@ struct OzShape {
 };
 
 // C (M x N2 real, leading dim ldc in doubles) (+)= recombined slice products.
-// As: S planes of M x K2 int8 (row-major, ld lda); Bt: S planes of N2 x K2 int8.
+// As: S tiled planes of M x K2 int8 (depth lda of the whole plane; the base
+// already points at this launch's depth chunk); Bt: S tiled planes of N2 x K2.
 template <int S, int WM, int WN, int WGM, int WGN, int MINW>
 __global__ void __launch_bounds__(64 * WGM * WGN, MINW) ozaki_gemm_kernel(
     const int8_t *__restrict__ As, uint64_t a_plane, uint64_t lda, const int8_t *__restrict__ Bt, uint64_t b_plane,
@@ -156,21 +169,30 @@ __global__ void __launch_bounds__(64 * WGM * WGN, MINW) ozaki_gemm_kernel(
 #pragma unroll
       for (int j = 0; j < Z::FB; ++j) acc[d][i][j] = int4_t{0, 0, 0, 0};
 
-  // Lane i of a chunk fills LDS slot i of it: row i / 4 of the chunk's 16,
-  // physical 16-byte column i % 4, which holds logical column (i % 4) ^ (row / 2 % 4).
-  const int crow = lane >> 2, ccol = (lane & 3) ^ ((lane >> 3) & 3);
-  auto issue = [&](int st, uint64_t k0) {
-    for (int c = wave; c < Z::CA + Z::CB; c += Z::NW) {
-      const bool isA = c < Z::CA;
-      const int cc = isA ? c : c - Z::CA;
-      const int rows = isA ? BM : BN;
-      const int p = cc / (rows / 16), g = cc % (rows / 16);
-      const int r = g * 16 + crow;
-      const uint64_t grow = isA ? min(m0 + r, M - 1) : min(n0 + r, N2 - 1);  // rows past the end: any valid row
-      const int8_t *src = isA ? As + p * a_plane + grow * lda : Bt + p * b_plane + grow * ldb;
-      int4_t *dst = isA ? &sA[st][(p * rows + g * 16) * 4] : &sB[st][(p * rows + g * 16) * 4];
-      __builtin_amdgcn_global_load_lds((const void *)(src + k0 + ccol * 16),
-                                       (void __attribute__((address_space(3))) *)dst, 16, 0, 0);
+  // A chunk of the tiled planes is one wave instruction: lane i copies its
+  // 16-byte slot i into LDS slot i.  The planes hold whole tiles of rows (the
+  // rows past M or N2 are padding whose products land only in outputs that are
+  // not stored), so no row is clamped: a chunk's address is a wave-uniform base
+  // plus the lane's 16 i, and issuing a chunk costs no per-lane arithmetic.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  const uint32_t off = 16 * lane;
+  // Row groups of 16 rows are lda / 64 chunks of 1 KiB apart; depth step kt is
+  // chunk kt of its row group.
+  auto issue = [&](int st, uint64_t kt) {
+#pragma unroll
+    for (int i = 0; i < (Z::CA + Z::CB + Z::NW - 1) / Z::NW; ++i) {
+      const int c = wv + Z::NW * i;
+      if (c < Z::CA + Z::CB) {
+        const bool isA = c < Z::CA;
+        const int cc = isA ? c : c - Z::CA;
+        const int rows = isA ? BM : BN;
+        const int p = cc / (rows / 16), g = cc % (rows / 16);
+        const int8_t *base = isA ? As + p * a_plane + ((m0 >> 4) + g) * (lda << 4) + (kt << 10)
+                                 : Bt + p * b_plane + ((n0 >> 4) + g) * (ldb << 4) + (kt << 10);
+        int4_t *dst = isA ? &sA[st][(p * rows + g * 16) * 4] : &sB[st][(p * rows + g * 16) * 4];
+        __builtin_amdgcn_global_load_lds((const void *)(base + off), (void __attribute__((address_space(3))) *)dst, 16,
+                                         0, 0);
+      }
     }
   };
 
@@ -181,7 +203,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN, MINW) ozaki_gemm_kernel(
   __syncthreads();
   for (uint64_t kt = 0; kt < nk; ++kt) {
     const int cur = (NST == 2) ? (int)(kt & 1) : 0;
-    if (NST == 2 && kt + 1 < nk) issue(cur ^ 1, (kt + 1) * BK);  // streams in during the MFMAs below
+    if (NST == 2 && kt + 1 < nk) issue(cur ^ 1, kt + 1);  // streams in during the MFMAs below
     // Fragments (16x16x64 i8): lane l holds row / column l % 16, k bytes 16 (l / 16) .. +15.
     // All B slices stay in registers; A slices stream through, one at a time.
     int4_t fb[S][Z::FB];
@@ -204,7 +226,7 @@ __global__ void __launch_bounds__(64 * WGM * WGN, MINW) ozaki_gemm_kernel(
     }
     if (NST == 1 && kt + 1 < nk) {
       __syncthreads();  // everyone is done reading the only stage
-      issue(0, (kt + 1) * BK);
+      issue(0, kt + 1);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -276,14 +298,14 @@ __global__ void __launch_bounds__(512) ozaki_gemm_staged(
     const uint64_t ga = m0 + ar;
 #pragma unroll
     for (int p = 0; p < S; ++p)
-      pa[p] = (ga < M) ? *reinterpret_cast<const int4_t *>(As + p * a_plane + ga * lda + k0 + ac * 16) : zero;
+      pa[p] = (ga < M) ? *reinterpret_cast<const int4_t *>(As + p * a_plane + tiled_at(ga, k0 + ac * 16, lda)) : zero;
 #pragma unroll
     for (int i = 0; i < BL; ++i) {
       const int c = tid + kGemmThreads * i;
       const int p = c / (BN * 4), r = (c % (BN * 4)) >> 2, ch = c & 3;
       const uint64_t gb = n0 + r;
       pb[i] = (c < S * BN * 4 && gb < N2)
-                  ? *reinterpret_cast<const int4_t *>(Bt + p * b_plane + gb * ldb + k0 + ch * 16)
+                  ? *reinterpret_cast<const int4_t *>(Bt + p * b_plane + tiled_at(gb, k0 + ch * 16, ldb))
                   : zero;
     }
   };
@@ -407,6 +429,12 @@ void launch_ozaki(hipStream_t st, const int8_t *As, uint64_t ap, uint64_t lda, c
 // Number of 8-bit slices of the split-precision GEMM (default 5).
 static int g_split_slices = 5;
 
+// Rows of a slice plane: whole 128-row tiles (every block tile is at most 128
+// rows of A and 128 rows of B^T), so the GEMM kernels read padding rows past
+// the end instead of clamping row indices.  The padding rows are never
+// written; their products reach only outputs past M or N2, which are not stored.
+static uint64_t plane_rows(uint64_t rows) { return (rows + 127) / 128 * 128; }
+
 extern "C" void qtnh_tune_split(int slices) { g_split_slices = std::max(1, std::min(slices, kMaxSlices)); }
 
 extern "C" void qtnh_tune_split_tile(int shape) { g_split_tile = (shape >= 0 && shape <= 3) ? shape : 0; }
@@ -424,7 +452,7 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
   const int S = g_split_slices;
   const uint64_t K2 = ((2 * K + 63) / 64) * 64;  // padded real depth (zeros beyond 2K)
   const uint64_t N2 = 2 * N;
-  const uint64_t a_plane = M * K2, b_plane = N2 * K2;
+  const uint64_t a_plane = plane_rows(M) * K2, b_plane = plane_rows(N2) * K2;
   const uint64_t need = S * (a_plane + b_plane) + 4 * (M + N2) + 512;
   if (scratch_bytes < need) return set_error(ctx, QTNH_EINVAL, "split: scratch too small");
   if (M > 0x7fffffffull || N > 0x7fffffffull) return set_error(ctx, QTNH_EUNSUPPORTED, "split: too many rows");
@@ -450,14 +478,16 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
   const uint64_t chunk = 8192;
   if (((M + 127) / 128) * ((N2 + 63) / 64) > 0x7fffffffull)
     return set_error(ctx, QTNH_EUNSUPPORTED, "split: grid too large");
+  // Depth chunk k0 (a multiple of 64) starts at chunk k0 / 64 of every row
+  // group of the tiled planes: byte 16 k0.
   for (uint64_t k0 = 0; k0 < K2; k0 += chunk) {
     const uint64_t kc = std::min(chunk, K2 - k0);
     const int accumulate = k0 > 0;
     switch (S) {
 #define QTNH_SPLIT_CASE(s)                                                                                            \
   case s:                                                                                                            \
-    launch_ozaki<s>(ctx->stream, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, (double *)C, N2, M, N2, kc,     \
-                    accumulate);                                                                                     \
+    launch_ozaki<s>(ctx->stream, As + 16 * k0, a_plane, K2, Bs + 16 * k0, b_plane, K2, ea, eb, (double *)C, N2, M,   \
+                    N2, kc, accumulate);                                                                             \
     break;
       QTNH_SPLIT_CASE(1)
       QTNH_SPLIT_CASE(2)
@@ -467,8 +497,8 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
       QTNH_SPLIT_CASE(6)
       QTNH_SPLIT_CASE(7)
       default:
-        launch_ozaki<8>(ctx->stream, As + k0, a_plane, K2, Bs + k0, b_plane, K2, ea, eb, (double *)C, N2, M, N2, kc,
-                        accumulate);
+        launch_ozaki<8>(ctx->stream, As + 16 * k0, a_plane, K2, Bs + 16 * k0, b_plane, K2, ea, eb, (double *)C, N2, M,
+                        N2, kc, accumulate);
         break;
 #undef QTNH_SPLIT_CASE
     }
@@ -482,7 +512,7 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c1
 uint64_t split_scratch_bytes(uint64_t M, uint64_t N, uint64_t K) {
   const int S = g_split_slices;
   const uint64_t K2 = ((2 * K + 63) / 64) * 64;
-  return S * (M * K2 + 2 * N * K2) + 4 * (M + 2 * N) + 512;
+  return S * (plane_rows(M) * K2 + plane_rows(2 * N) * K2) + 4 * (M + 2 * N) + 512;
 }
 
 }  // namespace qtnh_hip

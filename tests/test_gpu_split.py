@@ -191,10 +191,11 @@ def test_split_small_problem_is_exact_fp64():
     assert np.array_equal(host(cs).view(np.uint64), host(cf).view(np.uint64))
 
 
-def test_split_block_tile_does_not_change_bits():
-    """Every block tile of the split GEMM sums the same exact integer products."""
+@pytest.mark.parametrize("m,k,n", [(384, 512, 320), (400, 520, 330), (200, 5000, 100)])
+def test_split_block_tile_does_not_change_bits(m, k, n):
+    """Every block tile of the split GEMM sums the same exact integer products:
+    whole tiles, ragged edges with a zero-padded depth, and two depth chunks."""
     rng = np.random.default_rng(77)
-    m, k, n = 384, 512, 320
     A = rng.standard_normal(m * k) + 1j * rng.standard_normal(m * k)
     B = rng.standard_normal(k * n) + 1j * rng.standard_normal(k * n)
     da, db = [m, k], [k, n]
