@@ -49,6 +49,20 @@ def test_amplitude_matches_state_vector(rows, cols, depth):
     assert abs(got[0] - want) <= 1e-12 * max(abs(want), 2.0 ** (-n / 2))
 
 
+def test_best_of_sampled_orders_matches_state_vector():
+    # plan_best keeps a sampled greedy order (not the seed's own) on these
+    # networks; its amplitude is the state vector's all the same
+    for rows, cols, depth, seed in ((4, 4, 10, 7), (3, 5, 12, 8)):
+        n = rows * cols
+        c = ops.Circuit(n).grid(rows, cols, depth, 2000 + seed)
+        bits = np.random.default_rng(seed).integers(0, 2, n)
+        want, _ = statevector_amplitude(c, bits)
+        net = ops.Network.amplitude(c, bits)
+        chosen = net.plan_best(seed, 26, 32)
+        got = net.run()
+        assert abs(got[0] - want) <= 1e-12 * max(abs(want), 2.0 ** (-n / 2)), chosen
+
+
 def test_order_independence():
     c = ops.Circuit(12).grid(3, 4, 10, 5)
     bits = [1, 0] * 6

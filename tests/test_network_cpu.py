@@ -98,3 +98,58 @@ def test_disconnected_parts_end_in_an_outer_product():
     net.add([5], [7], np.zeros(5))
     net.plan()
     assert sorted(net.result_dims()) == [2, 5]
+
+
+def _step_nodes(net):
+    import ctypes
+    out = []
+    for s in range(net.info()["steps"]):
+        a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().qtnh_net_step(net.handle, s, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                             ctypes.byref(d)))
+        out.append((a.value, b.value, c.value))
+    return out
+
+
+def test_plan_best_keeps_the_fewest_flops_of_its_tries():
+    c = ops.Circuit(25).grid(5, 5, 12, 4)
+    bits = np.random.default_rng(4).integers(0, 2, 25)
+    first = ops.Network.amplitude(c, bits).plan(9, 26).info()
+    prev = None
+    for k in (1, 4, 16, 64):
+        net = ops.Network.amplitude(c, bits)
+        seed = net.plan_best(9, 26, k)
+        i = net.info()
+        assert 9 <= seed < 9 + k
+        if k == 1:
+            # try 0 is the deterministic greedy order of the seed
+            assert seed == 9 and i == first and _step_nodes(net) == _step_nodes(ops.Network.amplitude(c, bits).plan(9, 26))
+        assert i["flops"] <= first["flops"] and i["max_rank"] <= 26
+        if prev is not None:
+            assert i["flops"] <= prev  # the tries of a larger K include those of a smaller one
+        prev = i["flops"]
+        # a complete order: every tensor consumed once, one result left
+        used = set()
+        for a, b, cc in _step_nodes(net):
+            assert a not in used and b not in used and a < cc and b < cc
+            used.update((a, b))
+        assert len(used) == i["tensors"] + i["steps"] - 1
+    assert prev < first["flops"]  # sampled orders find a cheaper plan on this network
+
+
+def test_plan_best_is_deterministic():
+    c = ops.Circuit(24).grid(4, 6, 12, 11)
+    bits = np.random.default_rng(11).integers(0, 2, 24)
+    runs = []
+    for _ in range(3):
+        net = ops.Network.amplitude(c, bits)
+        seed = net.plan_best(100, 26, 48)
+        runs.append((seed, net.info(), _step_nodes(net)))
+    assert runs[0] == runs[1] == runs[2]
+
+
+def test_plan_best_rank_cap():
+    c = ops.Circuit(16).grid(4, 4, 12, 2)
+    with pytest.raises(_lib.QtnhError) as e:
+        ops.Network.amplitude(c).plan_best(0, 6, 16)
+    assert e.value.status == _lib.QTNH_EUNSUPPORTED
