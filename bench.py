@@ -374,11 +374,14 @@ def bench_c3_gemm(torch, ops, _lib, n, seed):
     cs = ops.empty(n * n)
     flops = 8.0 * n ** 3
     out = {"M": n, "N": n, "K": n, "GFLOP": flops / 1e9}
+    # Steady state: 3 untimed runs (the clocks settle over the first few
+    # milliseconds of matrix-core work), then 10 timed, for both precisions.
     for name, prec, dst in (("fp64", _lib.PREC_FP64, c64), ("split", _lib.PREC_SPLIT, cs)):
-        ops.contract(a, dims, b, dims, wires, precision=prec, out=dst)
+        for _ in range(3):
+            ops.contract(a, dims, b, dims, wires, precision=prec, out=dst)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = 5
+        reps = 10
         e0.record()
         for _ in range(reps):
             _, _, eta = ops.contract(a, dims, b, dims, wires, precision=prec, out=dst)

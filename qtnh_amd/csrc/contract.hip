@@ -615,13 +615,32 @@ static void gemm_layout(const Plan &P, const uint64_t *dA, const uint64_t *dB, i
   int q = 0;
   for (int i : P.open1) L.pA[i] = (uint16_t)q++;
   for (int i : P.cA) L.pA[i] = (uint16_t)q++;
-  q = 0;
-  if (L.split || L.splitk) {
-    for (int i : P.open2) L.pB[i] = (uint16_t)q++;
-    for (int i : P.cB) L.pB[i] = (uint16_t)q++;
+  // B: [open2, closed] for split-K; for split precision whichever of the
+  // two layouts B already has ([open2, closed] or [closed, open2]: the split
+  // reads either in place), else [closed, open2]; FP64 GEMM [closed, open2].
+  auto layout_b = [&](bool open_first) {
+    int qq = 0;
+    if (open_first) {
+      for (int i : P.open2) L.pB[i] = (uint16_t)qq++;
+      for (int i : P.cB) L.pB[i] = (uint16_t)qq++;
+    } else {
+      for (int i : P.cB) L.pB[i] = (uint16_t)qq++;
+      for (int i : P.open2) L.pB[i] = (uint16_t)qq++;
+    }
+    bool id = true;
+    for (int i = 0; i < P.rB; ++i) id = id && L.pB[i] == i;
+    return id;
+  };
+  L.b_cols = !L.splitk;
+  if (L.splitk) {
+    layout_b(true);
+  } else if (L.split) {
+    if (layout_b(true))
+      L.b_cols = false;
+    else
+      layout_b(false);
   } else {
-    for (int i : P.cB) L.pB[i] = (uint16_t)q++;
-    for (int i : P.open2) L.pB[i] = (uint16_t)q++;
+    layout_b(false);
   }
   L.idA = L.idB = L.idC = true;
   for (int i = 0; i < P.rA; ++i) L.idA = L.idA && L.pA[i] == i;
@@ -715,7 +734,7 @@ qtnh_status launch_contract(qtnh_ctx_t ctx, const c128 *A, int rA, const uint64_
     st = launch_zgemm_splitk(ctx, Ap, Bp, Cn, P.M, P.N, P.K, L.kchunks, (c128 *)(ws + L.offS));
     if (err_out) *err_out = 0.0;
   } else if (L.split) {
-    st = launch_zgemm_split(ctx, Ap, Bp, Cn, P.M, P.N, P.K, ws + L.offS, L.sbytes, err_out);
+    st = launch_zgemm_split(ctx, Ap, Bp, L.b_cols, Cn, P.M, P.N, P.K, ws + L.offS, L.sbytes, err_out);
   } else {
     st = launch_zgemm_f64(ctx, Ap, Bp, Cn, P.M, P.N, P.K, P.K, P.N, P.N,
                           L.zchunks > 1 ? (c128 *)(ws + L.offS) : nullptr);

@@ -118,6 +118,7 @@ ContractRoute contract_route(int rA, const uint64_t *dA, int rB, const uint64_t 
 // offC, then split scratch, split-K partials or depth-chunk partials at offS.
 struct GemmRoute {
   bool splitk = false, split = false;
+  bool b_cols = true;  // B' is [closed, open2] (K x N); else [open2, closed]
   bool idA = true, idB = true, idC = true;
   std::vector<uint16_t> pA, pB, pC;
   std::vector<uint64_t> dn;
@@ -208,10 +209,11 @@ uint64_t splitk_chunk_len(uint64_t K, uint64_t chunks);
 // (false when a dim is not a power of two).
 bool permute_source_bits(int rank, const uint64_t *dims, const uint16_t *ptup, int *nb, int8_t *sbit);
 
-// Split-precision complex GEMM C[MxN] = A[MxK] * Bt[NxK]^T (all contiguous,
-// row-major) on INT8 matrix cores; scratch of split_scratch_bytes(M, N, K).
-qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *Bt, c128 *C, uint64_t M, uint64_t N,
-                               uint64_t K, void *scratch, uint64_t scratch_bytes, double *err_out);
+// Split-precision complex GEMM C[MxN] = A[MxK] * B (all contiguous, row-major)
+// on INT8 matrix cores, B given as B[KxN] (b_cols) or as Bt[NxK]; scratch of
+// split_scratch_bytes(M, N, K).
+qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *B, bool b_cols, c128 *C, uint64_t M,
+                               uint64_t N, uint64_t K, void *scratch, uint64_t scratch_bytes, double *err_out);
 uint64_t split_scratch_bytes(uint64_t M, uint64_t N, uint64_t K);
 int split_slices();
 
