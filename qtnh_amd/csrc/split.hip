@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(kThreads) split_rows_kernel(const c128 *__rest
 // of B.  First the columns' largest part magnitudes, by atomic maximum over
 // blocks of 64 columns x kColRows rows on the bit patterns of non-negative
 // doubles (their order is that of the integers), from zero.
-constexpr int kColRows = 64;
+constexpr int kColRows = 256;
+constexpr int kColBatch = 16;  // loads in flight per thread
 
 __global__ void zero_u64_kernel(unsigned long long *__restrict__ p, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -234,18 +235,20 @@ __global__ void __launch_bounds__(kThreads) col_max_kernel(const c128 *__restric
   const uint64_t k0 = (uint64_t)blockIdx.y * kColRows + (threadIdx.x >> 6);
   double m[kParts] = {0.0, 0.0, 0.0};
   if (j < N) {
-    // Each wave reads every kWaves-th row of the block's rows: kPer
-    // coalesced 1 KiB row segments, all issued before they are used.
-    c128 v[kPer];
+    // Each wave reads every kWaves-th row of the block's rows: coalesced
+    // 1 KiB row segments, kColBatch of them issued before they are used.
+    for (int i0 = 0; i0 < kPer; i0 += kColBatch) {
+      c128 v[kColBatch];
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const uint64_t k = k0 + (uint64_t)i * kWaves;
-      v[i] = k < K ? B[k * N + j] : c128{0.0, 0.0};
+      for (int i = 0; i < kColBatch; ++i) {
+        const uint64_t k = k0 + (uint64_t)(i0 + i) * kWaves;
+        v[i] = k < K ? B[k * N + j] : c128{0.0, 0.0};
+      }
+#pragma unroll
+      for (int i = 0; i < kColBatch; ++i)
+#pragma unroll
+        for (int t = 0; t < kParts; ++t) m[t] = fmax(m[t], fabs(part_of(v[i], t)));
     }
-#pragma unroll
-    for (int i = 0; i < kPer; ++i)
-#pragma unroll
-      for (int t = 0; t < kParts; ++t) m[t] = fmax(m[t], fabs(part_of(v[i], t)));
   }
 #pragma unroll
   for (int t = 0; t < kParts; ++t) red[t][threadIdx.x >> 6][threadIdx.x & 63] = m[t];
