@@ -49,14 +49,28 @@ typedef double double2_t __attribute__((ext_vector_type(2)));
 // used when the lowest gate bit is >= 3, so that the 64 lanes of each load
 // cover whole 128-byte lines (with lower gate bits the lanes of one load are
 // strided inside lines, and the next load of the same lines must hit in cache).
+//
+// 3-qubit gates read their 64 coefficients from an LDS copy (made once per
+// workgroup) at each use: the 128 doubles do not fit the scalar registers, and
+// reloading them from the argument block per use is slower (measured 5.25 ->
+// 5.57 TB/s at 2^28 elements).  The empty asm with a memory clobber at the top
+// of each group keeps the compiler from hoisting the loop-invariant LDS reads
+// into registers (it did: 320 VGPRs).
 template <int KB, bool NT>
 __global__ void __launch_bounds__(kGateThreads) gate_bits_kernel(c128 *__restrict__ psi, GateArgs<KB> args,
                                                                  uint64_t n_groups) {
   constexpr int K = 1 << KB;
   constexpr int U = KB <= 2 ? 2 : 1;
+  constexpr bool kLdsCoef = KB == 3;
+  __shared__ double2_t gl[kLdsCoef ? K * K : 1];
+  if constexpr (kLdsCoef) {
+    for (int i = threadIdx.x; i < K * K; i += kGateThreads) gl[i] = double2_t{args.g[i].re, args.g[i].im};
+    __syncthreads();
+  }
   double2_t *p2 = reinterpret_cast<double2_t *>(psi);
   const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
   for (uint64_t g0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; g0 < n_groups; g0 += stride * U) {
+    if constexpr (kLdsCoef) asm volatile("" ::: "memory");
     uint64_t base[U];
     double2_t in[U][K];
 #pragma unroll
@@ -81,7 +95,8 @@ __global__ void __launch_bounds__(kGateThreads) gate_bits_kernel(c128 *__restric
 #pragma unroll
         for (int i = 0; i < K; ++i) {
           const c128 x = {in[u][i].x, in[u][i].y};
-          const c128 pr = cmul(x, args.g[i * K + o]);
+          const c128 gg = kLdsCoef ? c128{gl[i * K + o].x, gl[i * K + o].y} : args.g[i * K + o];
+          const c128 pr = cmul(x, gg);
           ar = ar + pr.re;
           ai = ai + pr.im;
         }
@@ -243,6 +258,11 @@ __global__ void __launch_bounds__(kGateThreads) gate_lanes_kernel(c128 *__restri
   // Every lane of a wave takes part in the shuffles: the loop bounds are
   // wave-uniform (n_rows is a multiple of 64 and so is the stride).
   for (uint64_t t0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; t0 < n_rows; t0 += stride * U) {
+    // LDS coefficients are read at each use: without this clobber the
+    // compiler hoists the loop-invariant reads into registers (218 VGPRs for
+    // a 3-qubit gate with two high bits, 2 waves per SIMD; 69 with it, and
+    // 4.6 -> 6.2 TB/s at 2^28 elements).
+    if constexpr (!kRegCoef) asm volatile("" ::: "memory");
     double2_t x[U][H];
     uint64_t e[U];
 #pragma unroll
@@ -327,12 +347,14 @@ qtnh_status launch_gate_lanes(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *
     args.src_lbits[i] = (uint8_t)lb;
     args.src_h[i] = (uint8_t)h;
   }
-  // Grid (measured at 2^28 and 2^30 elements): 1- and 2-qubit gates do best
-  // with one row per thread, so that the workgroups in flight cover one
-  // contiguous stretch of the state; 3-qubit gates (3 waves per SIMD or fewer)
-  // need each thread's U rows in flight together (one pass).
+  // Grid (measured at 2^28 and 2^30 elements): most 1- and 2-qubit gates do
+  // best with one row per thread, so that the workgroups in flight cover one
+  // contiguous stretch of the state; 3-qubit gates need each thread's U rows
+  // in flight together (one pass).
   const uint64_t rows = n >> KH;
-  constexpr uint64_t kRowsPerThread = KB <= 2 ? 1 : (KH >= 2 ? 1 : 4 >> KH);
+  // Two rows per thread for 2-qubit gates on two lane bits (5.27 -> 5.63 TB/s
+  // at 2^28 elements; one row was faster for the other 1- and 2-qubit shapes).
+  constexpr uint64_t kRowsPerThread = KB == 2 && KH == 0 ? 2 : KB <= 2 ? 1 : (KH >= 2 ? 1 : 4 >> KH);
   unsigned blocks = grid_for((rows + kRowsPerThread - 1) / kRowsPerThread, kGateThreads, kMaxStreamBlocks);
   hipLaunchKernelGGL((gate_lanes_kernel<KB, KH>), dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, rows);
   return check_hip(ctx, hipGetLastError(), "gate launch");

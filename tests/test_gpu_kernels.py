@@ -203,17 +203,27 @@ def test_swap_rejects_unequal_dims():
 
 
 # ------------------------------------------------------------------ gates
+# Placements that reach every gate kernel shape: all gate bits among the lowest
+# 6 (lane exchange, registers), some low and some high (lane exchange, LDS
+# coefficients, one or two high bits), all high (one thread per group).
+_GATE_PLACEMENTS = {1: [[13], [0]], 2: [[13, 12], [13, 0], [0, 13], [1, 2]],
+                    3: [[13, 12, 11], [13, 12, 0], [13, 0, 5], [0, 13, 11], [0, 1, 2], [7, 13, 3]]}
+
+
 @pytest.mark.parametrize("k", [1, 2, 3])
 def test_gate_bitexact_binary(k):
     pyr = random.Random(k)
     rank = 14
     dims = [2] * rank
-    for trial in range(6):
+    fixed = _GATE_PLACEMENTS[k]
+    for trial in range(6 + len(fixed)):
         axes = pyr.sample(range(rank), k)
         if trial == 0:
             axes = list(range(rank - k, rank))  # lowest bits
         if trial == 1:
             axes = list(range(k))               # highest bits
+        if trial >= 6:
+            axes = fixed[trial - 6]
         x = rnd(1 << rank, trial)
         g = rnd(4 ** k, 100 + trial)
         y = ops.gate_apply_(dev(x), dims, axes, g)
