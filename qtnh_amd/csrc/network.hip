@@ -173,6 +173,14 @@ struct qtnh_net_s {
   // Grouped-launch descriptor of each direct power-of-two step, without its
   // device pointers and launch position (filled in per batch placement).
   std::vector<GDesc> gdesc;
+  // Device copy of leaf_data: uploaded by the first blocking run on a device
+  // and read in place by later ones (leaf values cannot change once added;
+  // adding a leaf drops the copy).
+  void *dev_leaves = nullptr;
+  int dev_leaves_device = -1;
+  ~qtnh_net_s() {
+    if (dev_leaves) (void)hipFree(dev_leaves);
+  }
 };
 
 namespace qtnh_hip {
@@ -530,6 +538,33 @@ struct Where {
   c128 *result = nullptr;
 };
 
+// The network's host leaves on ctx's device: uploaded once (a blocking copy)
+// and kept by the network.  *out is null when it has no host leaves.
+qtnh_status resident_leaves(qtnh_ctx_t ctx, qtnh_net_s &net, c128 **out) {
+  *out = nullptr;
+  if (net.leaf_elems == 0) return QTNH_OK;
+  if (!net.dev_leaves || net.dev_leaves_device != ctx->device) {
+    if (net.dev_leaves) (void)hipFree(net.dev_leaves);
+    net.dev_leaves = nullptr;
+    net.dev_leaves_device = -1;
+    void *p = nullptr;
+    const uint64_t bytes = net.leaf_elems * sizeof(c128);
+    qtnh_status st = check_hip(ctx, hipMalloc(&p, bytes), "network leaf copy alloc");
+    if (st) return st;
+    st = check_hip(ctx, hipMemcpyAsync(p, net.leaf_data.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+                   "network leaf upload");
+    if (!st) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "network leaf upload sync");
+    if (st) {
+      (void)hipFree(p);
+      return st;
+    }
+    net.dev_leaves = p;
+    net.dev_leaves_device = ctx->device;
+  }
+  *out = (c128 *)net.dev_leaves;
+  return QTNH_OK;
+}
+
 c128 *node_ptr(const qtnh_net_s &net, Where block, int x) {
   if ((size_t)x < net.n_leaves)
     return net.nodes[x].ext ? (c128 *)net.nodes[x].ext : block.leaves + net.nodes[x].offset;
@@ -611,9 +646,6 @@ uint64_t gdesc_blocks(const GDesc &d) { return (d.nC + kGThreads - 1) / kGThread
 namespace qtnh_hip {
 namespace {
 
-// Descriptors per upload-and-launch range of an eager batch pass.
-constexpr size_t kDescChunk = 2048;
-
 // Whether step s of network n runs in its step index's grouped launches
 // (given the running workspace of the step index's earlier grouped steps).
 bool grouped_step(const qtnh_net_s &n, int prec, size_t s, uint64_t sum) {
@@ -661,11 +693,17 @@ struct Wave {
   std::vector<const qtnh_net_s *> nets;
   std::vector<int> precs;  // each network's precision when the batch was laid out
   std::vector<Where> blocks;
-  std::vector<double> leaves;  // host copy of every leaf, packed as on the device
+  std::vector<double> leaves;  // host copy of every leaf, packed as on the device (own_leaves)
+  bool own_leaves = true;      // false: every network's leaves are read in place
   std::vector<GDesc> descs;    // host copy, when built in full (prepared batches)
   const GDesc *hdesc = nullptr;  // host copy in use (descs or the context's pinned staging)
   size_t ndesc = 0;
   std::vector<size_t> starts;  // descriptors of step index s: [starts[s], starts[s + 1])
+  // Descriptor of network i's step s: dk[dk_off[i] + s] (direct power-of-two
+  // steps only), with its first block in its step index's launch dfirst[k].
+  std::vector<uint32_t> dk;
+  std::vector<size_t> dk_off;
+  std::vector<uint64_t> dfirst;
   std::vector<Gather> gathers;
   c128 *base = nullptr, *res = nullptr;
   char *gws = nullptr;
@@ -675,6 +713,7 @@ struct Wave {
   Gather *dgath = nullptr;
   // Grouped GEMM-route steps: host lists (all step indices) and ranges per step index.
   std::vector<std::vector<char>> grouped;  // [network][step]
+  std::vector<std::vector<int>> singles;   // [step index]: networks whose step runs alone
   std::vector<PermDesc> pin, pout;
   std::vector<ZGroupDesc> zg[2];  // block width 64, 128
   std::vector<ZSumDesc> zsum;
@@ -695,7 +734,9 @@ struct Wave {
   // Placement of every network in `mem`, the leaf image, the descriptor count
   // of each step index and the result gather table; with `with_descs` also
   // every descriptor (else fill_descs builds them step range by step range).
-  void layout(const qtnh_net_t *ns, int count, char *mem, bool with_descs) {
+  // With `resident`, network i's leaves are read in place from resident[i]
+  // (when not null) instead of from the batch's own leaf image.
+  void layout(const qtnh_net_t *ns, int count, char *mem, bool with_descs, c128 *const *resident = nullptr) {
     nets.assign(ns, ns + count);
     for (auto *n : nets) precs.push_back(n->precision);
     uint64_t elems = 0;
@@ -711,25 +752,65 @@ struct Wave {
     base = (c128 *)mem;
     res = base + elems;
     gws = (char *)(res + round16(res_elems));
-    leaves.assign(2 * std::max<uint64_t>(leaf_elems, 1), 0.0);
+    own_leaves = !resident;
+    if (own_leaves) leaves.assign(2 * std::max<uint64_t>(leaf_elems, 1), 0.0);
     uint64_t loff = 0, aoff = leaf_elems;
-    for (auto *n : nets) {
-      blocks.push_back({base + loff, base + aoff});
-      memcpy(leaves.data() + 2 * loff, n->leaf_data.data(), n->leaf_elems * sizeof(c128));
+    for (size_t i = 0; i < nets.size(); ++i) {
+      const auto *n = nets[i];
+      blocks.push_back({resident && resident[i] ? resident[i] : base + loff, base + aoff});
+      if (own_leaves) memcpy(leaves.data() + 2 * loff, n->leaf_data.data(), n->leaf_elems * sizeof(c128));
       loff += round16(n->leaf_elems);
       aoff += round16(n->arena_elems);
     }
+    // Descriptor positions, network by network (each network's steps are
+    // contiguous in memory): descriptors of a step index in network order, and
+    // each descriptor's first block from the step templates' result sizes.
     starts.assign(steps + 1, 0);
-    for (auto *n : nets)
+    dk_off.assign(nets.size(), 0);
+    size_t ndk = 0;
+    for (size_t i = 0; i < nets.size(); ++i) {
+      const auto *n = nets[i];
+      dk_off[i] = ndk;
+      ndk += n->steps.size();
       for (size_t s = 0; s < n->steps.size(); ++s) starts[s + 1] += n->steps[s].direct && n->steps[s].binary;
+    }
     for (size_t s = 0; s < steps; ++s) starts[s + 1] += starts[s];
     ndesc = starts[steps];
+    dk.assign(ndk, 0);
+    dfirst.assign(ndesc, 0);
+    {
+      std::vector<size_t> next(starts.begin(), starts.end() - 1);
+      for (size_t i = 0; i < nets.size(); ++i) {
+        const auto *n = nets[i];
+        for (size_t s = 0; s < n->steps.size(); ++s) {
+          if (!n->steps[s].direct || !n->steps[s].binary) continue;
+          const size_t k = next[s]++;
+          dk[dk_off[i] + s] = (uint32_t)k;
+          dfirst[k] = gdesc_blocks(n->gdesc[s]);  // block count; prefix sums below
+        }
+      }
+      for (size_t s = 0; s < steps; ++s) {
+        uint64_t first = 0;
+        for (size_t k = starts[s]; k < starts[s + 1]; ++k) {
+          const uint64_t b = dfirst[k];
+          dfirst[k] = first;
+          first += b;
+        }
+      }
+    }
     if (with_descs) {
       descs.resize(ndesc);
-      fill_descs(0, steps, descs.data());
+      fill_descs(descs.data());
       hdesc = descs.data();
     }
     layout_grouped();
+    // Steps that run alone (not direct power-of-two, not grouped), per step index.
+    singles.assign(steps, std::vector<int>());
+    for (size_t i = 0; i < nets.size(); ++i)
+      for (size_t s = 0; s < nets[i]->steps.size(); ++s) {
+        const auto &st = nets[i]->steps[s];
+        if (!(st.direct && st.binary) && !grouped[i][s]) singles[s].push_back((int)i);
+      }
     uint64_t dst = 0;
     for (size_t i = 0; i < nets.size(); ++i) {
       const auto &n = *nets[i];
@@ -850,18 +931,28 @@ struct Wave {
   }
 
   // Descriptors of step indices [s0, s1) into out[starts[s0] .. starts[s1]).
-  void fill_descs(size_t s0, size_t s1, GDesc *out) const {
-    for (size_t s = s0; s < s1; ++s) {
-      size_t k = starts[s];
-      uint64_t first = 0;
-      for (size_t i = 0; i < nets.size(); ++i) {
+  // Every direct-step descriptor, network by network, networks split over up
+  // to kPlanThreads host threads (measured: 8-13 ms for the 93k descriptors
+  // of a 256-network batch when built step index by step index on one thread,
+  // which touches every network per step index).
+  void fill_descs(GDesc *out) const {
+    const int nt = (int)std::max<size_t>(1, std::min<size_t>({nets.size(), (size_t)kPlanThreads,
+                                                               (size_t)std::max(1u, std::thread::hardware_concurrency()),
+                                                               (ndesc + 4095) / 4096}));
+    auto work = [&](int t) {
+      for (size_t i = t; i < nets.size(); i += nt) {
         const auto &n = *nets[i];
-        if (s >= n.steps.size() || !n.steps[s].direct || !n.steps[s].binary) continue;
-        out[k] = make_gdesc(n, blocks[i], s, first);
-        first += gdesc_blocks(out[k]);
-        ++k;
+        for (size_t s = 0; s < n.steps.size(); ++s) {
+          if (!n.steps[s].direct || !n.steps[s].binary) continue;
+          const size_t k = dk[dk_off[i] + s];
+          out[k] = make_gdesc(n, blocks[i], s, dfirst[k]);
+        }
       }
-    }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
   }
 
   static uint64_t pad256(uint64_t b) { return (b + 255) & ~255ull; }
@@ -892,12 +983,39 @@ struct Wave {
     d_sk = (SkDesc *)p;
   }
 
+  // Bytes from the gather table to the end of the descriptor region (laid out
+  // as on the device): what upload_packed stages.
+  uint64_t tail_bytes() const { return desc_bytes() - pad256(ndesc * sizeof(GDesc)); }
+
+  // The gather table and the grouped GEMM-route descriptors, packed into
+  // pinned staging at their device layout and sent with one asynchronous copy
+  // (each copy from pageable memory waits for the stream and stages through
+  // the driver: ~6 ms for the seven tables of a 256-network batch).  The
+  // staging must stay untouched until the stream has passed the copy.
+  qtnh_status upload_packed(qtnh_ctx_t ctx, char *staging) {
+    char *const d0 = (char *)dgath;
+    auto put = [&](const void *dst, const void *src, size_t bytes) {
+      if (bytes) memcpy(staging + ((const char *)dst - d0), src, bytes);
+    };
+    put(dgath, gathers.data(), gathers.size() * sizeof(Gather));
+    put(d_pin, pin.data(), pin.size() * sizeof(PermDesc));
+    put(d_pout, pout.data(), pout.size() * sizeof(PermDesc));
+    put(d_zg[0], zg[0].data(), zg[0].size() * sizeof(ZGroupDesc));
+    put(d_zg[1], zg[1].data(), zg[1].size() * sizeof(ZGroupDesc));
+    put(d_zsum, zsum.data(), zsum.size() * sizeof(ZSumDesc));
+    put(d_sk, sk.data(), sk.size() * sizeof(SkDesc));
+    return check_hip(ctx, hipMemcpyAsync(d0, staging, tail_bytes(), hipMemcpyHostToDevice, ctx->stream),
+                     "batch descriptor upload");
+  }
+
   // Leaves, the gather table, the grouped GEMM-route descriptors and (when
   // built in full) the direct-step descriptors to the device (stream-ordered).
   qtnh_status upload(qtnh_ctx_t ctx) {
-    qtnh_status st = check_hip(
-        ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
-        "batch leaf upload");
+    qtnh_status st = QTNH_OK;
+    if (own_leaves)
+      st = check_hip(
+          ctx, hipMemcpyAsync(base, leaves.data(), leaf_elems * sizeof(c128), hipMemcpyHostToDevice, ctx->stream),
+          "batch leaf upload");
     if (!st && !descs.empty())
       st = check_hip(ctx, hipMemcpyAsync(ddesc, descs.data(), descs.size() * sizeof(GDesc), hipMemcpyHostToDevice,
                                          ctx->stream),
@@ -960,10 +1078,8 @@ struct Wave {
       }
       qtnh_status st = enqueue_grouped(ctx, s);
       if (st) return st;
-      for (size_t i = 0; i < nets.size(); ++i) {
-        const auto &n = *nets[i];
-        if (s >= n.steps.size() || (n.steps[s].direct && n.steps[s].binary) || grouped[i][s]) continue;
-        st = run_step(ctx, n, blocks[i], s, gws, ws_bytes, precs[i]);
+      for (int i : singles[s]) {
+        st = run_step(ctx, *nets[i], blocks[i], s, gws, ws_bytes, precs[i]);
         if (st) return st;
       }
     }
@@ -1041,6 +1157,11 @@ qtnh_status qtnh_net_add_tensor(qtnh_net_t net, int rank, const uint64_t *dims, 
   net->leaf_elems = x.offset + n;
   net->leaf_data.resize(2 * net->leaf_elems, 0.0);
   memcpy(net->leaf_data.data() + 2 * x.offset, data, n * sizeof(c128));
+  if (net->dev_leaves) {
+    (void)hipFree(net->dev_leaves);
+    net->dev_leaves = nullptr;
+    net->dev_leaves_device = -1;
+  }
   if (id) *id = (int)net->nodes.size();
   net->nodes.push_back(x);
   net->n_leaves = net->nodes.size();
@@ -1253,10 +1374,11 @@ qtnh_status qtnh_net_run(qtnh_ctx_t ctx, qtnh_net_t net, double *out, uint64_t o
   if (st) return st;
   Where block = {(c128 *)w, (c128 *)w + round16(net->leaf_elems)};
   char *ws = (char *)((c128 *)w + elems);
-  st = check_hip(ctx, hipMemcpyAsync(block.leaves, net->leaf_data.data(), net->leaf_elems * sizeof(c128),
-                                     hipMemcpyHostToDevice, ctx->stream),
-                 "net_run leaf upload");
+  // Leaves are read in place from the network's device copy.
+  c128 *dl = nullptr;
+  st = resident_leaves(ctx, *net, &dl);
   if (st) return st;
+  if (dl) block.leaves = dl;
   for (size_t s = 0; s < net->steps.size(); ++s) {
     st = run_step(ctx, *net, block, s, ws, net->ws_bytes, net->precision);
     if (st) return st;
@@ -1325,33 +1447,33 @@ qtnh_status qtnh_net_run_batch(qtnh_ctx_t ctx, int count, const qtnh_net_t *nets
       ++w1;
     }
     ++waves;
+    // Leaves are read in place from each network's device copy.
+    std::vector<c128 *> resident(w1 - w0, nullptr);
+    for (int i = w0; i < w1 && !st; ++i) st = resident_leaves(ctx, *nets[i], &resident[i - w0]);
+    if (st) return st;
     void *wp = nullptr;
     st = workspace(ctx, bytes, &wp);
     if (st) return st;
     Wave w;
-    w.layout(nets + w0, w1 - w0, (char *)wp, false);
+    w.layout(nets + w0, w1 - w0, (char *)wp, false, resident.data());
+    // Pinned staging: the direct-step descriptors, then the other tables.
+    const uint64_t gbytes = ((std::max<size_t>(w.ndesc, 1) * sizeof(GDesc)) + 255) & ~255ull;
     st = grow_buffer(ctx, &ctx->desc, &ctx->desc_bytes, w.desc_bytes());
-    if (!st) st = grow_pinned(ctx, std::max<size_t>(w.ndesc, 1) * sizeof(GDesc));
+    if (!st) st = grow_pinned(ctx, gbytes + w.tail_bytes());
     if (st) return st;
     w.place_descriptors(ctx->desc);
-    st = w.upload(ctx);
-    // Descriptors are built, copied (from pinned memory, asynchronously) and
-    // launched a range of step indices at a time, so the GPU starts on the
-    // first steps while the host builds the rest.  Each range's staging is
-    // written once and read by its copy before the wave's closing sync.
+    st = w.upload_packed(ctx, (char *)ctx->pinned + gbytes);
+    // Descriptors are built into pinned staging (host threads), copied with
+    // one asynchronous copy, then every step is launched.  The staging is read
+    // by its copy before the wave's closing sync.
     GDesc *hp = (GDesc *)ctx->pinned;
     w.hdesc = hp;
-    for (size_t s0 = 0, s1; !st && s0 < w.steps; s0 = s1) {
-      s1 = s0 + 1;
-      while (s1 < w.steps && w.starts[s1] - w.starts[s0] < kDescChunk) ++s1;
-      w.fill_descs(s0, s1, hp);
-      const size_t b = w.starts[s0], e = w.starts[s1];
-      if (e > b)
-        st = check_hip(ctx, hipMemcpyAsync(w.ddesc + b, hp + b, (e - b) * sizeof(GDesc), hipMemcpyHostToDevice,
-                                           ctx->stream),
-                       "batch descriptor upload");
-      if (!st) st = w.enqueue_steps(ctx, s0, s1);
+    if (!st && w.ndesc) {
+      w.fill_descs(hp);
+      st = check_hip(ctx, hipMemcpyAsync(w.ddesc, hp, w.ndesc * sizeof(GDesc), hipMemcpyHostToDevice, ctx->stream),
+                     "batch descriptor upload");
     }
+    if (!st) st = w.enqueue_steps(ctx, 0, w.steps);
     if (!st) st = w.enqueue_gather(ctx);
     if (!st) st = w.fetch(ctx, outs + w0);
     if (st) return st;
