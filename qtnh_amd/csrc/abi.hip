@@ -316,6 +316,9 @@ qtnh_status qtnh_ctx_create(int device, void *stream, qtnh_ctx_t *out) {
   }
   // The context runs on the caller's stream; NULL is the device's default stream.
   if (st == QTNH_OK) ctx->stream = (hipStream_t)stream;
+  if (st == QTNH_OK) st = check_hip(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "side stream");
+  if (st == QTNH_OK) st = check_hip(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming), "event");
+  if (st == QTNH_OK) st = check_hip(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming), "event");
   if (st == QTNH_OK) st = check_hip(ctx, hipHostMalloc((void **)&ctx->host_scalar, 64), "hipHostMalloc");
   if (st == QTNH_OK) st = check_hip(ctx, hipMalloc((void **)&ctx->dev_scalar, 2048 * sizeof(double)), "hipMalloc scalar");
   // Device address of the no-allocation entry point's scratch array (once per device).
@@ -347,6 +350,12 @@ qtnh_status qtnh_ctx_destroy(qtnh_ctx_t ctx) {
   // Teardown is best effort: errors here cannot be reported to anyone useful.
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->desc) (void)hipFree(ctx->desc);
   if (ctx->red) (void)hipFree(ctx->red);

@@ -30,6 +30,10 @@ struct qtnh_ctx_s {
   // Growable pinned host buffer staging descriptor uploads (asynchronous copies).
   void *pinned = nullptr;
   uint64_t pinned_bytes = 0;
+  // Second stream (default priority) for work that overlaps the caller's
+  // stream inside one call, forked and joined with the two events.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // Fused-circuit tile shape chosen by the autotuner, per qubit count (0: unset).
   int8_t circ_tb[64] = {0};
   int8_t circ_low[64] = {0};

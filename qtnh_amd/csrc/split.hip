@@ -217,9 +217,9 @@ __global__ void __launch_bounds__(kThreads) split_rows_kernel(const c128 *__rest
 // B read in place as K x N (row-major, the FP64 GEMM's layout), giving the
 // planes split_rows_kernel gives for the rows of B^T, with no transposed copy
 // of B.  First the columns' largest part magnitudes, by atomic maximum over
-// blocks of 64 columns x kColRows rows on the bit patterns of non-negative
-// doubles (their order is that of the integers), from zero.
-constexpr int kColRows = 256;
+// blocks of rows on the bit patterns of non-negative doubles (their order is
+// that of the integers), from zero.
+constexpr int kColRows = 64;
 constexpr int kColBatch = 16;  // loads in flight per thread
 
 __global__ void zero_u64_kernel(unsigned long long *__restrict__ p, uint64_t n) {
@@ -229,38 +229,28 @@ __global__ void zero_u64_kernel(unsigned long long *__restrict__ p, uint64_t n) 
 
 __global__ void __launch_bounds__(kThreads) col_max_kernel(const c128 *__restrict__ B, uint64_t K, uint64_t N,
                                                            unsigned long long *__restrict__ cmax) {
-  constexpr int kWaves = kThreads / 64, kPer = kColRows / kWaves;
-  __shared__ double red[kParts][kWaves][64];
-  const uint64_t j = (uint64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const uint64_t k0 = (uint64_t)blockIdx.y * kColRows + (threadIdx.x >> 6);
+  // Block: kThreads consecutive columns (each row read as one contiguous
+  // 4 KiB run by its four waves) x kColRows rows, kColBatch loads in flight
+  // per thread; one atomic per part and column.
+  const uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t k0 = (uint64_t)blockIdx.y * kColRows;
+  if (j >= N) return;  // no barriers here
   double m[kParts] = {0.0, 0.0, 0.0};
-  if (j < N) {
-    // Each wave reads every kWaves-th row of the block's rows: coalesced
-    // 1 KiB row segments, kColBatch of them issued before they are used.
-    for (int i0 = 0; i0 < kPer; i0 += kColBatch) {
-      c128 v[kColBatch];
+  for (int i0 = 0; i0 < kColRows; i0 += kColBatch) {
+    c128 v[kColBatch];
 #pragma unroll
-      for (int i = 0; i < kColBatch; ++i) {
-        const uint64_t k = k0 + (uint64_t)(i0 + i) * kWaves;
-        v[i] = k < K ? B[k * N + j] : c128{0.0, 0.0};
-      }
-#pragma unroll
-      for (int i = 0; i < kColBatch; ++i)
-#pragma unroll
-        for (int t = 0; t < kParts; ++t) m[t] = fmax(m[t], fabs(part_of(v[i], t)));
+    for (int i = 0; i < kColBatch; ++i) {
+      const uint64_t k = k0 + (uint64_t)(i0 + i);
+      v[i] = k < K ? B[k * N + j] : c128{0.0, 0.0};
     }
+#pragma unroll
+    for (int i = 0; i < kColBatch; ++i)
+#pragma unroll
+      for (int t = 0; t < kParts; ++t) m[t] = fmax(m[t], fabs(part_of(v[i], t)));
   }
 #pragma unroll
-  for (int t = 0; t < kParts; ++t) red[t][threadIdx.x >> 6][threadIdx.x & 63] = m[t];
-  __syncthreads();
-  if (threadIdx.x < 64 && j < N) {
-#pragma unroll
-    for (int t = 0; t < kParts; ++t) {
-#pragma unroll
-      for (int w = 1; w < kWaves; ++w) m[t] = fmax(m[t], red[t][w][threadIdx.x]);
-      if (m[t] > 0.0) atomicMax(cmax + t * N + j, (unsigned long long)__double_as_longlong(m[t]));
-    }
-  }
+  for (int t = 0; t < kParts; ++t)
+    if (m[t] > 0.0) atomicMax(cmax + t * N + j, (unsigned long long)__double_as_longlong(m[t]));
 }
 
 // Digits of B (K x N) into the B^T planes of the three parts.  A block takes
@@ -389,14 +379,15 @@ __device__ __forceinline__ OzTile oz_tile(const OzArgs &a, int BM, int BN) {
   return r;
 }
 
-// Recombines one output of part t from its S diagonal sums and stores it.
+// Recombines one output of part t from its S diagonal sums (row scale
+// 2^er fr, column scale 2^ec fc) and stores it.
 template <int S>
-__device__ __forceinline__ void oz_store(const OzArgs &a, int t, uint64_t gm, uint64_t gn, const int *accd) {
+__device__ __forceinline__ void oz_store(const OzArgs &a, int t, uint64_t gm, uint64_t gn, const int *accd, double fr,
+                                         int er, double fc, int ec) {
   double sum = 0.0;
 #pragma unroll
   for (int d = S - 1; d >= 0; --d) sum += ldexp((double)accd[d], 8 * (2 * S - 2 - d));
-  const uint64_t row = (uint64_t)t * a.M + gm, col = (uint64_t)t * a.N + gn;
-  const double v = ldexp(sum * (a.fa[row] * a.fb[col]), a.ea[row] + a.eb[col]);
+  const double v = ldexp(sum * (fr * fc), er + ec);
   const uint64_t MN = a.M * a.N, e = gm * a.N + gn;
   if (t == 2 && a.combine) {
     double t3 = v;
@@ -410,9 +401,31 @@ __device__ __forceinline__ void oz_store(const OzArgs &a, int t, uint64_t gm, ui
 }
 
 // 16x16 i32 result layout: lane l holds column l % 16 of rows 4 (l / 16) + v.
+// The scales of the lane's 4 FA rows and FB columns are read once (the
+// stores in between would otherwise make the compiler reload them per
+// output: 4 scattered loads per output).
 template <int S, int FA, int FB>
 __device__ __forceinline__ void oz_epilogue(const OzArgs &a, const OzTile &tl, int wm, int wn, int lane,
                                             const int4_t (&acc)[S][FA][FB]) {
+  const int t = tl.part;
+  double fr[FA][4], fc[FB];
+  int er[FA][4], ec[FB];
+#pragma unroll
+  for (int i = 0; i < FA; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const uint64_t gm = tl.m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+      const uint64_t r = (uint64_t)t * a.M + (gm < a.M ? gm : 0);
+      fr[i][v] = a.fa[r];
+      er[i][v] = a.ea[r];
+    }
+#pragma unroll
+  for (int j = 0; j < FB; ++j) {
+    const uint64_t gn = tl.n0 + wn + 16 * j + (lane & 15);
+    const uint64_t c = (uint64_t)t * a.N + (gn < a.N ? gn : 0);
+    fc[j] = a.fb[c];
+    ec[j] = a.eb[c];
+  }
 #pragma unroll
   for (int i = 0; i < FA; ++i)
 #pragma unroll
@@ -425,7 +438,7 @@ __device__ __forceinline__ void oz_epilogue(const OzArgs &a, const OzTile &tl, i
           int accd[S];
 #pragma unroll
           for (int d = 0; d < S; ++d) accd[d] = acc[d][i][j][v];
-          oz_store<S>(a, tl.part, gm, gn, accd);
+          oz_store<S>(a, t, gm, gn, accd, fr[i][v], er[i][v], fc[j], ec[j]);
         }
       }
 }
@@ -646,34 +659,49 @@ void launch_ozaki(hipStream_t st, const OzArgs &a, int nparts) {
   }
 }
 
+// A's planes on `stream`, B's on `side`: independent memory-bound passes
+// that run together (forked from and joined back into `stream` by the two
+// events, which a graph capture of `stream` follows).
 template <int S>
-void split_planes(hipStream_t stream, const c128 *A, const c128 *B, bool b_cols, uint64_t M, uint64_t N, uint64_t K,
-                  int8_t *As, uint64_t a_plane, int8_t *Bs, uint64_t b_plane, uint64_t Kp, ScaleOut sa, ScaleOut sb,
-                  unsigned long long *cmax) {
+hipError_t split_planes(hipStream_t stream, hipStream_t side, hipEvent_t fork, hipEvent_t join, const c128 *A,
+                        const c128 *B, bool b_cols, uint64_t M, uint64_t N, uint64_t K, int8_t *As, uint64_t a_plane,
+                        int8_t *Bs, uint64_t b_plane, uint64_t Kp, ScaleOut sa, ScaleOut sb,
+                        unsigned long long *cmax) {
+  hipError_t e = hipEventRecord(fork, stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((split_rows_kernel<S>), dim3((unsigned)M), dim3(kThreads), 0, stream, A, M, K, As, a_plane,
                      (uint64_t)S * a_plane, Kp, sa);
   if (b_cols) {
     // B as K x N: the columns' maxima, then digits read in place.
     hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)std::min<uint64_t>((kParts * N + 255) / 256, 65536)), dim3(256),
-                       0, stream, cmax, kParts * N);
-    hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((N + 63) / 64), (unsigned)((K + kColRows - 1) / kColRows)),
-                       dim3(kThreads), 0, stream, B, K, N, cmax);
+                       0, side, cmax, kParts * N);
+    hipLaunchKernelGGL(col_max_kernel, dim3((unsigned)((N + kThreads - 1) / kThreads),
+                                            (unsigned)((K + kColRows - 1) / kColRows)),
+                       dim3(kThreads), 0, side, B, K, N, cmax);
     hipLaunchKernelGGL((split_cols_kernel<S>), dim3((unsigned)((N + 63) / 64), (unsigned)((K + 63) / 64)),
-                       dim3(kThreads), 0, stream, B, K, N, cmax, Bs, b_plane, (uint64_t)S * b_plane, Kp, sb);
+                       dim3(kThreads), 0, side, B, K, N, cmax, Bs, b_plane, (uint64_t)S * b_plane, Kp, sb);
   } else {
     // B^T as N x K: one block per row.
-    hipLaunchKernelGGL((split_rows_kernel<S>), dim3((unsigned)N), dim3(kThreads), 0, stream, B, N, K, Bs, b_plane,
+    hipLaunchKernelGGL((split_rows_kernel<S>), dim3((unsigned)N), dim3(kThreads), 0, side, B, N, K, Bs, b_plane,
                        (uint64_t)S * b_plane, Kp, sb);
   }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(join, side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
+  return e;
 }
 
 // The depth chunks of one split GEMM: parts 1 and 2 in one launch, then part
 // 3, which on the last chunk also forms C.
 template <int S>
-void split_gemm(hipStream_t stream, const c128 *A, const c128 *B, bool b_cols, c128 *C, uint64_t M, uint64_t N,
-                uint64_t K, int8_t *As, uint64_t a_plane, int8_t *Bs, uint64_t b_plane, uint64_t Kp, ScaleOut sa,
-                ScaleOut sb, unsigned long long *cmax, double *T, uint64_t chunk) {
-  split_planes<S>(stream, A, B, b_cols, M, N, K, As, a_plane, Bs, b_plane, Kp, sa, sb, cmax);
+hipError_t split_gemm(qtnh_ctx_t ctx, const c128 *A, const c128 *B, bool b_cols, c128 *C, uint64_t M, uint64_t N,
+                      uint64_t K, int8_t *As, uint64_t a_plane, int8_t *Bs, uint64_t b_plane, uint64_t Kp, ScaleOut sa,
+                      ScaleOut sb, unsigned long long *cmax, double *T, uint64_t chunk) {
+  hipStream_t stream = ctx->stream;
+  const hipError_t e = split_planes<S>(stream, ctx->side, ctx->ev_fork, ctx->ev_join, A, B, b_cols, M, N, K, As,
+                                       a_plane, Bs, b_plane, Kp, sa, sb, cmax);
+  if (e != hipSuccess) return e;
   OzArgs a;
   memset(&a, 0, sizeof(a));
   a.a_plane = a_plane;
@@ -704,6 +732,7 @@ void split_gemm(hipStream_t stream, const c128 *A, const c128 *B, bool b_cols, c
     a.combine = k0 + chunk >= Kp;
     launch_ozaki<S>(stream, a, 1);
   }
+  return hipGetLastError();
 }
 
 // Rows of a slice plane: whole 128-row tiles (every block tile is at most 128
@@ -788,11 +817,12 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *B, boo
     qtnh_status st = launch_zero(ctx, As, L.offT);
     if (st) return st;
   }
+  hipError_t e = hipSuccess;
   switch (S) {
-#define QTNH_SPLIT_CASE(s)                                                                                        \
-  case s:                                                                                                        \
-    split_gemm<s>(ctx->stream, A, B, b_cols, C, M, N, K, As, L.a_plane, Bs, L.b_plane, L.Kp, sa, sb, cmax, T, \
-                  kSplitChunk);                                                                                  \
+#define QTNH_SPLIT_CASE(s)                                                                                      \
+  case s:                                                                                                      \
+    e = split_gemm<s>(ctx, A, B, b_cols, C, M, N, K, As, L.a_plane, Bs, L.b_plane, L.Kp, sa, sb, cmax, T,     \
+                      kSplitChunk);                                                                            \
     break;
     QTNH_SPLIT_CASE(1)
     QTNH_SPLIT_CASE(2)
@@ -806,7 +836,7 @@ qtnh_status launch_zgemm_split(qtnh_ctx_t ctx, const c128 *A, const c128 *B, boo
     default:
       return set_error(ctx, QTNH_EINVAL, "split: %d slices", S);
   }
-  qtnh_status st = check_hip(ctx, hipGetLastError(), "split gemm launch");
+  qtnh_status st = check_hip(ctx, e, "split gemm launch");
   if (st) return st;
   if (err_out) *err_out = split_error_bound(S, K);
   return QTNH_OK;

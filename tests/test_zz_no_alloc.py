@@ -107,3 +107,30 @@ def test_noalloc_refuses_a_workspace_beyond_the_scratch():
     st, _ = call("qtnh_contract_c128", A, dA, B, dB, wires, _lib.PREC_FP64, out)
     assert st == 0
     torch.cuda.synchronize()
+
+
+def test_noalloc_split_captured_in_a_graph():
+    """The split-precision route forks its operand splitting onto the
+    context's second stream and joins it back: captured into a HIP graph
+    through the entry point that allocates nothing, replays equal the eager
+    result bitwise, with other work between replays."""
+    cs = torch.cuda.Stream()
+    with torch.cuda.stream(cs):
+        A, dA, B, dB, wires, nC = case(41, 8, 8, 8)
+        want = ops.empty(nC)
+        got = ops.empty(nC)
+        st, _ = call("qtnh_contract_c128_noalloc", A, dA, B, dB, wires, _lib.PREC_SPLIT, want)
+        assert st == 0
+    cs.synchronize()
+    g = torch.cuda.CUDAGraph()
+    print("noalloc split: capture", flush=True)
+    with torch.cuda.graph(g, stream=cs):
+        st, _ = call("qtnh_contract_c128_noalloc", A, dA, B, dB, wires, _lib.PREC_SPLIT, got)
+    assert st == 0
+    other = ops.fill_(ops.empty(1 << 16), 3)
+    for rep in range(3):
+        got.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        ops.scale_(other, 0.5)
+        assert same_bits(got, want), rep
