@@ -545,7 +545,11 @@ __device__ __forceinline__ void apply_reg_op(double2_t (&v)[16], uint32_t code, 
     case 29: reg_diag<2, 3, 0, 0>(v, g); break;
     case 61: reg_diag<2, 3, 1, 0>(v, g); break;
     case 93: reg_diag<2, 3, 2, 0>(v, g); break;
-    default: break;
+    // The host emits only the codes above (reg_code_ok).  With no default
+    // path, each leaf of the compiler's branch tree is a case, and the 16
+    // amplitudes are not copied to a merge register set on the way (32 64-bit
+    // moves per thread per op, measured in the gfx950 code).
+    default: __builtin_unreachable();
   }
 }
 
@@ -959,6 +963,14 @@ void free_compiled(qtnh_circuit_s &c) {
   c.retired.clear();
 }
 
+// Whether a register-phase op code is one of apply_reg_op's cases: a gate on
+// 1 or 2 register bits, or a diagonal on 0-2 of them, bits distinct.
+bool reg_code_ok(uint32_t code) {
+  const uint32_t kind = code & 1u, kb = (code >> 1) & 3u;
+  if (kb > 2 || (kind == 0 && kb == 0) || (code >> (3 + 2 * kb)) != 0) return false;
+  return kb < 2 || ((code >> 3) & 3u) != ((code >> 5) & 3u);
+}
+
 // Register phases of a 12-bit-tile run: consecutive ops whose tile bits fit 4,
 // padded with the highest free tile bits (threads then vary in the low bits).
 void plan_phases(std::vector<TileOp> &tops, uint32_t begin, uint32_t end, std::vector<PhaseDesc> &out) {
@@ -1099,6 +1111,9 @@ qtnh_status compile(qtnh_ctx_t ctx, qtnh_circuit_s &c) {
     if (r.tbits.size() == 12) plan_phases(tops, r.begin, r.end, phases);
     c.run_phases.push_back({p0, (uint32_t)phases.size()});
   }
+  for (const auto &ph : phases)
+    for (uint32_t i = ph.op_begin; !ph.lds && i < ph.op_end; ++i)
+      if (!reg_code_ok(tops[i].rcode)) return set_error(ctx, QTNH_EINVAL, "circuit: register op form (planner)");
   if (phases.empty()) phases.resize(1);
   qtnh_status st = check_hip(ctx, hipMalloc((void **)&c.d_ops, tops.size() * sizeof(TileOp)), "circuit alloc");
   if (!st) st = check_hip(ctx, hipMalloc((void **)&c.d_phases, phases.size() * sizeof(PhaseDesc)), "circuit alloc");
