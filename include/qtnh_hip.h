@@ -216,13 +216,19 @@ qtnh_status qtnh_net_destroy(qtnh_net_t net);
 /* Adds a leaf tensor (host data, row-major complex); *id (may be NULL) gets its node id. */
 qtnh_status qtnh_net_add_tensor(qtnh_net_t net, int rank, const uint64_t *dims, const int64_t *bonds,
                                 const double *data, int *id);
-/* The best of `tries` greedy plans: try 0 is qtnh_net_plan's order for
- * `seed`, try t >= 1 a sampled greedy order (random-greedy at low
- * temperature) drawn from seed + t.  Keeps the fewest flops (sum of 8 M N K)
- * among those whose intermediates stay within max_rank (ties: the lowest
- * try); *chosen_seed (may be NULL) gets seed + t of the kept try.  The tries
- * run on up to 16 host threads; the result does not depend on their number.
- * QTNH_EUNSUPPORTED when no try fits the cap. */
+/* The best of `tries` greedy plans by estimated time.  A plan's cost is the
+ * sum over its steps of 8 M N K flops + 1024 x the elements the step reads
+ * and writes (operands and result): flops + 64 x the bytes qtnh_net_info
+ * reports.  Try 0 is qtnh_net_plan's order for `seed`, try t >= 1 a sampled
+ * greedy order (random-greedy at low temperature) drawn from seed + t; each
+ * try contracts its last 10 tensors in their cheapest order (exhaustive
+ * search over subsets).  Keeps the least cost among the tries whose
+ * intermediates stay within max_rank (ties: the lowest try), and plans the
+ * kept try with the search over its last 12 tensors (never a higher cost);
+ * *chosen_seed (may be NULL) gets seed + t of the kept try.  Networks of at
+ * most 12 tensors get their cheapest order.  The tries run on up to 16 host
+ * threads; the result does not depend on their number.  QTNH_EUNSUPPORTED
+ * when no try fits the cap. */
 qtnh_status qtnh_net_plan_best(qtnh_net_t net, uint64_t seed, int max_rank, int tries, uint64_t *chosen_seed);
 /* Adds a leaf whose elements stay in caller device memory (row-major complex,
  * read in place by every run; it must stay valid and unchanged until the runs

@@ -3,7 +3,11 @@
 The planner is pinned by: determinism from the seed; every step contracting
 two live tensors that share a bond; the flop and byte counts of hand-sized
 contractions; the rank cap; and the amplitude network of a circuit holding
-one tensor per op plus the |0> and <bits| vectors.
+one tensor per op plus the |0> and <bits| vectors.  The exhaustive search
+over the last tensors of a best-of plan is pinned by a brute-force forward
+search over every pairwise contraction sequence of small random networks,
+with and without a binding rank cap, under the planner's cost (flops plus a
+weight per element read and written).
 """
 import numpy as np
 import pytest
@@ -111,30 +115,133 @@ def _step_nodes(net):
     return out
 
 
-def test_plan_best_keeps_the_fewest_flops_of_its_tries():
+# The objective of qtnh_net_plan_best: flops + 1024 x the elements each step
+# reads and writes = flops + 64 x the bytes (16 per element) info() reports.
+ELEM_COST = 1024
+
+
+def plan_cost(info):
+    return info["flops"] + ELEM_COST / 16 * info["bytes"]
+
+
+def test_plan_best_keeps_the_cheapest_of_its_tries():
     c = ops.Circuit(25).grid(5, 5, 12, 4)
     bits = np.random.default_rng(4).integers(0, 2, 25)
     first = ops.Network.amplitude(c, bits).plan(9, 26).info()
-    prev = None
+    greedy_steps = _step_nodes(ops.Network.amplitude(c, bits).plan(9, 26))
+    found = {}
     for k in (1, 4, 16, 64):
         net = ops.Network.amplitude(c, bits)
         seed = net.plan_best(9, 26, k)
         i = net.info()
         assert 9 <= seed < 9 + k
         if k == 1:
-            # try 0 is the deterministic greedy order of the seed
-            assert seed == 9 and i == first and _step_nodes(net) == _step_nodes(ops.Network.amplitude(c, bits).plan(9, 26))
-        assert i["flops"] <= first["flops"] and i["max_rank"] <= 26
-        if prev is not None:
-            assert i["flops"] <= prev  # the tries of a larger K include those of a smaller one
-        prev = i["flops"]
+            # try 0: the seed's greedy order, its last tensors in the order the
+            # exhaustive search picks (the greedy prefix is unchanged)
+            steps = _step_nodes(net)
+            assert seed == 9 and i["steps"] == first["steps"]
+            assert steps[:len(steps) - 11] == greedy_steps[:len(steps) - 11]
+        # never costlier than the seed's greedy order, within the cap
+        assert plan_cost(i) <= plan_cost(first) and i["max_rank"] <= 26
+        found[k] = plan_cost(i)
         # a complete order: every tensor consumed once, one result left
         used = set()
         for a, b, cc in _step_nodes(net):
             assert a not in used and b not in used and a < cc and b < cc
             used.update((a, b))
         assert len(used) == i["tensors"] + i["steps"] - 1
-    assert prev < first["flops"]  # sampled orders find a cheaper plan on this network
+    assert found[1] < plan_cost(first)  # the search over the last tensors alone saves here
+    assert found[64] < found[1]  # and sampled orders more
+
+
+def _random_network(rng, n, n_bonds, n_open):
+    """n tensors; n_bonds bonds between random distinct pairs, n_open open legs
+    on random tensors; every dimension 2 or 3.  Returns (net, legs, dims)
+    where legs[i] lists tensor i's bond labels."""
+    legs = [[] for _ in range(n)]
+    dims = {}
+    for b in range(n_bonds):
+        i, j = rng.choice(n, 2, replace=False)
+        legs[i].append(b)
+        legs[j].append(b)
+        dims[b] = int(rng.integers(2, 4))
+    for b in range(n_bonds, n_bonds + n_open):
+        legs[int(rng.integers(n))].append(b)
+        dims[b] = int(rng.integers(2, 4))
+    net = ops.Network()
+    for i in range(n):
+        if not legs[i]:  # a scalar tensor: give it an open leg of its own
+            b = len(dims)
+            legs[i].append(b)
+            dims[b] = 2
+        d = [dims[b] for b in legs[i]]
+        net.add(d, legs[i], rng.standard_normal(int(np.prod(d))))
+    return net, legs, dims
+
+
+def _least_cost(legs, dims, cap):
+    """Least plan cost (8 x the product of the dimensions of the pair's bonds,
+    plus ELEM_COST x the elements of both operands and the result, per step)
+    over every sequence of pairwise contractions (outer products included)
+    whose intermediates have at most cap axes, by a memoised forward search
+    over the partitions of the tensors into groups."""
+    from functools import lru_cache
+    n = len(legs)
+
+    def open_bonds(group):
+        seen = {}
+        for i in group:
+            for b in legs[i]:
+                seen[b] = seen.get(b, 0) + 1
+        return frozenset(b for b, c in seen.items() if c == 1)
+
+    @lru_cache(maxsize=None)
+    def best(state):
+        if len(state) == 1:
+            return 0
+        groups = sorted(state, key=sorted)
+        out = None
+        for x in range(len(groups)):
+            for y in range(x + 1, len(groups)):
+                merged = groups[x] | groups[y]
+                if len(open_bonds(merged)) > cap:
+                    continue
+                ox, oy, om = open_bonds(groups[x]), open_bonds(groups[y]), open_bonds(merged)
+                size = lambda bonds: int(np.prod([dims[b] for b in bonds]))  # noqa: E731
+                f = 8 * size(ox | oy) + ELEM_COST * (size(ox) + size(oy) + size(om))
+                rest = best(frozenset(g for k, g in enumerate(groups) if k not in (x, y)) | {merged})
+                if rest is not None and (out is None or f + rest < out):
+                    out = f + rest
+        return out
+
+    return best(frozenset(frozenset([i]) for i in range(n)))
+
+
+# (seed, tensors, bonds, open legs, caps): cap 26 never binds; cap 4 leaves no
+# order for seeds 301, 302, 303 and 305; the last four caps bind (the least
+# cost within them exceeds the unbounded one: 1122640 against 1098064,
+# 921984 / 901712, 642408 / 621600, 1374048 / 1332864).
+SMALL_NETWORKS = [(300, 5, 10, 0, (26, 5, 4)), (301, 6, 12, 1, (26, 5, 4)), (302, 7, 14, 2, (26, 5, 4)),
+                  (303, 5, 10, 0, (26, 5, 4)), (304, 6, 12, 1, (26, 5, 4)), (305, 7, 14, 2, (26, 5, 4)),
+                  (429, 6, 9, 5, (26, 5)), (464, 6, 9, 3, (26, 4)), (513, 8, 11, 3, (26, 4)),
+                  (514, 8, 14, 3, (26, 5))]
+
+
+@pytest.mark.parametrize("seed,n,n_bonds,n_open,caps", SMALL_NETWORKS)
+def test_plan_best_small_networks_take_the_cheapest_order(seed, n, n_bonds, n_open, caps):
+    # At most 12 tensors: the whole order comes from the exhaustive search.
+    net, legs, dims = _random_network(np.random.default_rng(seed), n, n_bonds, n_open)
+    for cap in caps:
+        want = _least_cost(legs, dims, cap)
+        if want is None:
+            with pytest.raises(_lib.QtnhError):
+                net.plan_best(1, cap, 2)
+            continue
+        net.plan_best(1, cap, 2)
+        i = net.info()
+        assert plan_cost(i) == want
+        # max_rank counts the given tensors too; results stay within the cap
+        assert i["max_rank"] <= max(cap, max(len(x) for x in legs)) and i["steps"] == n - 1
 
 
 def test_plan_best_is_deterministic():
