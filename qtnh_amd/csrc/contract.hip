@@ -584,8 +584,13 @@ static bool use_direct(const Plan &P) {
 }
 
 // Few outputs, long sums (inner products, closing steps of a network): the
-// sums are split into ordered chunks over many workgroups.
-static bool use_splitk(const Plan &P) { return P.M * P.N <= (1ull << 14); }
+// sums are split into ordered chunks over many workgroups.  Each output reads
+// its own rows of A' and B', 2 M N K elements in all, where the GEMM route
+// reads (M + N) K once but computes whole 64-wide tiles: per term, 32 M N
+// bytes at ~5.5 TB/s against 8 x 64 x 64 flops at ~67 TFLOP/s plus 16 (M + N)
+// bytes.  Split-K is the faster up to about 64 outputs (measured: 32 x 32
+// outputs over 2^20 terms took 1.2 ms in split-K, reading 32 GiB of rows).
+static bool use_splitk(const Plan &P) { return P.M * P.N <= 64; }
 
 // Chunks per output of the split-K route; independent of the device.
 static uint64_t splitk_chunks(uint64_t MN, uint64_t K) {

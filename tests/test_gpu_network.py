@@ -243,7 +243,8 @@ def test_grouped_gemm_route_steps_bitwise(use_graph):
     nets, refs = [], []
     shapes = [(9, 6, 11, True, False),   # 512 x 64 x 2048: depth chunks, both operands permuted
               (8, 8, 6, True, False),    # 256 x 256 x 64: plain tiles
-              (4, 4, 14, False, True),   # 16 x 16 x 16384: split-K
+              (3, 3, 14, False, True),   # 8 x 8 x 16384: split-K
+              (4, 4, 14, False, True),   # 16 x 16 x 16384: depth chunks
               (9, 6, 11, False, True),
               (7, 9, 8, True, True)]
     for i, (mb, nb, kb, af, bf) in enumerate(shapes * 2):

@@ -72,7 +72,7 @@ def same_bits(x, y):
 ROUTES = [
     ("gemm-fp64", _lib.PREC_FP64, (9, 9, 9), (8, 8, 6)),
     ("split", _lib.PREC_SPLIT, (9, 9, 9), (8, 8, 7)),
-    ("split-k", _lib.PREC_FP64, (6, 6, 12), (5, 5, 11)),
+    ("split-k", _lib.PREC_FP64, (3, 3, 14), (2, 2, 13)),
     ("direct", _lib.PREC_FP64, (14, 2, 2), (8, 2, 2)),
 ]
 
