@@ -157,7 +157,7 @@ struct qtnh_net_s {
     uint64_t ws = 0;
     uint64_t ws64 = 0;       // workspace of the step's FP64 route
     bool groupable = false;  // FP64 GEMM-route step small enough for the batch's grouped launches
-    uint64_t M = 0, N = 0;  // GEMM shape of the route (open A x open B)
+    uint64_t M = 0, N = 0, K = 0;  // GEMM shape of the route (open A x open B, closed)
     double flops = 0, bytes = 0;
   };
   size_t n_leaves = 0;
@@ -192,11 +192,16 @@ uint64_t block_elems(const qtnh_net_s &n) { return round16(n.leaf_elems) + round
 
 GDesc gdesc_template(const qtnh_net_s &net, size_t s);
 
-// A network in split precision uses the split GEMM only for steps whose
-// 128 x 128 output tiles over M x 2N (the embedded real product) cover the
-// 256 CUs; narrower GEMMs (few tiles, long sums) are faster in FP64, where
-// depth chunks fill the chip, and exact.
-constexpr uint64_t kSplitMinTiles = 256;
+// A network in split precision uses the split GEMM only for steps where it
+// is the faster: 128 x 128 output tiles over M x 2N (the embedded real
+// product) that cover the 256 CUs, M and 2N of at least 512, and sums of at
+// least 256 terms.  Measured against the FP64 GEMM (operands in place):
+// M = 2^13, N = 2^12 and M = 2^10, N = 2^15: split 0.23-0.40x at K = 32-64,
+// 0.73-0.75x at K = 128, 1.14-1.19x at K = 256, 3.1x at 4096^3; M = 128 or 32:
+// 0.4-0.75x at every K up to 4096 (a 128-row tile mostly empty or digit
+// products too short to pay for the splitting and the three-buffer
+// recombination).  Other steps stay exact FP64.
+constexpr uint64_t kSplitMinTiles = 256, kSplitMinSide = 512, kSplitMinK = 256;
 
 // Batches run the GEMM-route steps of one step index across networks in
 // grouped launches when each is small (operands and result of at most 2^22
@@ -218,7 +223,9 @@ inline double step_cost(double flops, double elems) { return flops + kElemCost *
 
 int step_precision(const qtnh_net_s::Step &st, int precision) {
   if (precision != QTNH_PREC_SPLIT) return QTNH_PREC_FP64;
-  return ((st.M + 127) / 128) * ((2 * st.N + 127) / 128) >= kSplitMinTiles ? QTNH_PREC_SPLIT : QTNH_PREC_FP64;
+  const bool wins = ((st.M + 127) / 128) * ((2 * st.N + 127) / 128) >= kSplitMinTiles && st.M >= kSplitMinSide &&
+                    2 * st.N >= kSplitMinSide && st.K >= kSplitMinK;
+  return wins ? QTNH_PREC_SPLIT : QTNH_PREC_FP64;
 }
 
 // Route launch_contract takes for a step (default placement) at a precision.
@@ -305,6 +312,7 @@ void add_step(qtnh_net_s &net, qtnh_net_s::Step st, qtnh_net_s::Node r, Arena &a
   st.direct = R.direct;
   st.M = R.M;
   st.N = R.N;
+  st.K = R.K;
   st.ws = step_precision(st, net.precision) == QTNH_PREC_SPLIT ? step_route(net, st, QTNH_PREC_SPLIT).ws_bytes
                                                                 : R.ws_bytes;
   st.ws64 = R.ws_bytes;
