@@ -96,6 +96,15 @@ BitLayout bit_layout(int rank, const uint64_t *dims);
 qtnh_status launch_fill(qtnh_ctx_t ctx, c128 *buf, uint64_t n, uint64_t seed, int dist);
 qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
                            const uint16_t *ptup);
+// A square dense gate of gate_bits (1 or 2) term bits contracted into A with
+// its outputs on the lowest result bits, as the bit permutation sigma (A bit
+// -> C bit, the term bits to C bits 0 .. gate_bits-1) with each result row
+// formed in the tiles (reference term order and arithmetic); xb[t 2^gb + o]:
+// B offset of term t for output o.  nb: A's index bits.  *done false when the
+// shape does not fit the tiles (nothing launched; the caller takes another
+// kernel).  C must not overlap A.
+qtnh_status launch_permute_gate(qtnh_ctx_t ctx, const c128 *A, c128 *C, const c128 *B, int nb, const int *sigma,
+                                int gate_bits, const uint32_t *xb, bool *done);
 qtnh_status launch_swap(qtnh_ctx_t ctx, c128 *buf, int rank, const uint64_t *dims, int i, int j);
 qtnh_status launch_gate(qtnh_ctx_t ctx, c128 *state, int rank, const uint64_t *dims, int k,
                         const uint16_t *axes, const double *G);

@@ -153,6 +153,112 @@ permute_bits_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, PermBi
   }
 }
 
+// A square dense gate (2^NB terms, 2^NB outputs) contracted into a state with
+// its outputs on the lowest result bits, fused into the bit permutation that
+// carries the state's other bits to their result positions.  The permutation
+// sends the gate's term bits to result bits 0 .. NB-1, so a tile holds whole
+// result rows.  The load phase is permute_bits_kernel's; in the store phase a
+// thread takes the same result elements as there (consecutive lanes,
+// consecutive elements, so each store instruction writes whole lines), reads
+// the 2^NB terms of the element's row from the tile (lanes of one row read the
+// same slots) and forms the element: 0 + the sum over the terms in the
+// reference order (term t = 0, 1, ...) of separately rounded complex products,
+// contract_direct_rows's arithmetic bit for bit.  A thread's elements all have
+// output index tid mod 2^NB, so it loads its 2^NB coefficients once.
+template <int NB>
+struct GateRowArgs {
+  const c128 *B;
+  uint32_t xb[(1 << NB) * (1 << NB)];  // B offset of term t for output o, at t 2^NB + o
+};
+
+template <int U, typename Off, int NB>
+__global__ void __launch_bounds__(kPermThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
+permute_gate_kernel(const c128 *__restrict__ src, c128 *__restrict__ dst, PermBitsArgs args, GateRowArgs<NB> gr,
+                    uint64_t n_tiles) {
+  constexpr uint32_t T = 1u << U;
+  constexpr int N = 1 << NB;
+  constexpr int EPT = (int)(T / kPermThreads);
+  static_assert(T >= kPermThreads, "every thread takes elements of a tile");
+  extern __shared__ c128 tile[];
+  const uint32_t amask = (1u << args.a) - 1u;
+  const uint32_t bmask = (1u << args.b) - 1u;
+  constexpr Off kScale = sizeof(Off) == 4 ? (Off)sizeof(c128) : (Off)1;
+  // Tile element e = tid + 256 r: its source offset, its LDS slot in the load
+  // phase, its result offset, and the slot of its row's term 0 (the row's
+  // result-order index e - o; term t sits at tile index s2t(e - o) XOR s2t(t),
+  // and lds_slot is linear over XOR).
+  Off xo[EPT], yo[EPT];
+  uint32_t slots[EPT];
+#pragma unroll
+  for (int r = 0; r < EPT; ++r) {
+    const uint32_t e = threadIdx.x + r * kPermThreads;
+    xo[r] = (Off)((e & amask) | args.in_hi[e >> args.a]) * kScale;
+    yo[r] = (Off)((e & bmask) | args.out_hi[e >> args.b]) * kScale;
+    const uint32_t s = e & ~(uint32_t)(N - 1);
+    slots[r] = lds_slot(e, args.a) | (lds_slot(args.s2t_lo[s & 63u] | args.s2t_hi[s >> 6], args.a) << 16);
+  }
+  uint32_t sterm[N];
+  double2_t cf[N];
+  const int o = (int)(threadIdx.x & (N - 1));
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    sterm[t] = lds_slot(args.s2t_lo[t], args.a);
+    const c128 c = gr.B[gr.xb[t * N + o]];
+    cf[t] = double2_t{c.re, c.im};
+  }
+  auto at = [](const c128 *base, Off off) -> const double2_t * {
+    if (sizeof(Off) == 4) return reinterpret_cast<const double2_t *>(reinterpret_cast<const char *>(base) + off);
+    return reinterpret_cast<const double2_t *>(base + off);
+  };
+  auto at_mut = [](c128 *base, Off off) -> c128 * {
+    if (sizeof(Off) == 4) return reinterpret_cast<c128 *>(reinterpret_cast<char *>(base) + off);
+    return base + off;
+  };
+  double2_t *ltile = reinterpret_cast<double2_t *>(tile);
+  uint64_t g = blockIdx.x;
+  if (g >= n_tiles) return;  // uniform: the whole block leaves
+  uint64_t xb, yb;
+  tile_bases(args, g, xb, yb);
+  double2_t v[EPT];
+  {
+    const c128 *s = src + xb;
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) v[r] = load_d2<true>(at(s, xo[r]));
+  }
+  while (true) {
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) ltile[slots[r] & 0xffffu] = v[r];
+    __syncthreads();
+    const uint64_t gn = g + gridDim.x;
+    const bool more = gn < n_tiles;  // uniform
+    uint64_t xn = 0, yn = 0;
+    if (more) {
+      tile_bases(args, gn, xn, yn);
+      const c128 *s = src + xn;
+#pragma unroll
+      for (int r = 0; r < EPT; ++r) v[r] = load_d2<true>(at(s, xo[r]));
+    }
+    c128 *d = dst + yb;
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) {
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int t = 0; t < N; ++t) {
+        const double2_t x = ltile[(slots[r] >> 16) ^ sterm[t]];
+        double pr, pi;
+        cmul_parts(x.x, x.y, cf[t].x, cf[t].y, &pr, &pi);
+        ar = ar + pr;
+        ai = ai + pi;
+      }
+      store_d2<true>(at_mut(d, yo[r]), double2_t{ar, ai});
+    }
+    __syncthreads();
+    if (!more) break;
+    g = gn;
+    yb = yn;
+  }
+}
+
 // General dims: result element y -> source offset.  nd: result dims, jump:
 // source stride of the source axis placed at each result axis.
 struct PermGenArgs {
@@ -410,9 +516,11 @@ void launch_bits(hipStream_t stream, int num_cus, const c128 *src, c128 *dst, co
 // Times the candidate tile shapes for one signature on the caller's buffers and
 // returns the fastest; *wrote is set when dst already holds the result.  Every
 // shape is timed once, then the two fastest again (the better second timing
-// wins), so one noisy timing does not decide.
-qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sigma, const int *inv, int nb,
-                      PermTune *best, bool *wrote) {
+// wins), so one noisy timing does not decide.  launch(args, u, no, cfg) runs
+// one shape and returns false when the kernel does not take it.
+template <class Launch>
+qtnh_status tune_shapes(qtnh_ctx_t ctx, const int *sigma, const int *inv, int nb, Launch launch, PermTune *best,
+                        bool *wrote) {
   *wrote = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t err = hipEventCreate(&e0);
@@ -422,10 +530,10 @@ qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sig
     PermBitsArgs args;
     int u, no;
     if (err != hipSuccess || !plan_bits(sigma, inv, nb, cand, args, &u, &no)) return -1.0f;
-    launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cand);  // warm-up; writes the result
+    if (!launch(args, u, no, cand)) return -1.0f;  // warm-up; writes the result
     err = hipEventRecord(e0, ctx->stream);
     for (int r = 0; r < kPermTimedRuns && err == hipSuccess; ++r) {
-      launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cand);
+      launch(args, u, no, cand);
       err = hipGetLastError();
     }
     if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
@@ -462,7 +570,114 @@ qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sig
   if (e1) (void)hipEventDestroy(e1);
   return check_hip(ctx, err, "permute tuning");
 }
+
+qtnh_status tune_bits(qtnh_ctx_t ctx, const c128 *src, c128 *dst, const int *sigma, const int *inv, int nb,
+                      PermTune *best, bool *wrote) {
+  auto launch = [&](const PermBitsArgs &args, int u, int no, const PermTune &cfg) {
+    launch_bits(ctx->stream, ctx->num_cus, src, dst, args, u, no, nb, cfg);
+    return true;
+  };
+  return tune_shapes(ctx, sigma, inv, nb, launch, best, wrote);
+}
+
+// The tuned tile shape cached for a signature key, or (not capturing, large
+// enough) the fastest of the candidates timed with launch; else the default.
+// *wrote: the timing runs already wrote the result.
+template <class Launch>
+qtnh_status tuned_shape(qtnh_ctx_t ctx, const std::string &key, uint64_t n, const int *sigma, const int *inv, int nb,
+                        Launch launch, PermTune *cfg, bool *wrote) {
+  *wrote = false;
+  *cfg = g_perm;
+  if (!g_perm_auto || n < kPermAutoMin) return QTNH_OK;
+  {
+    std::lock_guard<std::mutex> lock(g_perm_mu);
+    auto it = g_perm_cache.find(key);
+    if (it != g_perm_cache.end()) {
+      *cfg = it->second;
+      return QTNH_OK;
+    }
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess) {
+    (void)hipGetLastError();  // not tuned: the query's error is not the caller's
+    return QTNH_OK;
+  }
+  if (cap != hipStreamCaptureStatusNone) return QTNH_OK;
+  qtnh_status st = tune_shapes(ctx, sigma, inv, nb, launch, cfg, wrote);
+  if (st) return st;
+  std::lock_guard<std::mutex> lock(g_perm_mu);
+  if (g_perm_cache.size() >= kPermCacheMax) g_perm_cache.clear();
+  g_perm_cache[key] = *cfg;
+  return QTNH_OK;
+}
+
+template <int NB, int U>
+void launch_gate_u(const PermLaunch &L, const PermBitsArgs &args, const GateRowArgs<NB> &gr) {
+  if (L.narrow)
+    hipLaunchKernelGGL((permute_gate_kernel<U, uint32_t, NB>), dim3(L.blocks), dim3(kPermThreads), L.lds, L.stream,
+                       L.src, L.dst, args, gr, L.n_tiles);
+  else
+    hipLaunchKernelGGL((permute_gate_kernel<U, uint64_t, NB>), dim3(L.blocks), dim3(kPermThreads), L.lds, L.stream,
+                       L.src, L.dst, args, gr, L.n_tiles);
+}
+
+// One launch of the fused gate kernel; false for tiles under 10 bits (the
+// kernels are built for the 10- to 12-bit tiles the tuned shapes use).
+template <int NB>
+bool launch_gate_rows(hipStream_t stream, int num_cus, const c128 *src, c128 *dst, const PermBitsArgs &args, int u,
+                      int no, int nb, const PermTune &cfg, const GateRowArgs<NB> &gr) {
+  const uint64_t n_tiles = 1ull << no;
+  const uint64_t cap = (uint64_t)std::max(1, num_cus) * (uint64_t)cfg.blocks_per_cu;
+  const unsigned blocks = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_tiles, cap), 1u << 30);
+  PermLaunch L{stream, blocks, (size_t)sizeof(c128) << u, src, dst, n_tiles, nb <= 28, true};
+  switch (u) {
+    case 10: launch_gate_u<NB, 10>(L, args, gr); return true;
+    case 11: launch_gate_u<NB, 11>(L, args, gr); return true;
+    case 12: launch_gate_u<NB, 12>(L, args, gr); return true;
+    default: return false;
+  }
+}
+
+template <int NB>
+qtnh_status permute_gate(qtnh_ctx_t ctx, const c128 *A, c128 *C, const c128 *B, int nb, const int *sigma,
+                         const uint32_t *xb, bool *done) {
+  *done = false;
+  int inv[64];
+  for (int p = 0; p < nb; ++p) inv[sigma[p]] = p;
+  GateRowArgs<NB> gr;
+  gr.B = B;
+  for (int i = 0; i < (1 << NB) * (1 << NB); ++i) gr.xb[i] = xb[i];
+  auto launch = [&](const PermBitsArgs &args, int u, int no, const PermTune &cfg) {
+    return launch_gate_rows<NB>(ctx->stream, ctx->num_cus, A, C, args, u, no, nb, cfg, gr);
+  };
+  std::string key((const char *)&ctx->device, sizeof(int));
+  key.push_back('G');
+  key.push_back((char)NB);
+  for (int p = 0; p < nb; ++p) key.push_back((char)sigma[p]);
+  PermTune cfg;
+  bool wrote = false;
+  qtnh_status st = tuned_shape(ctx, key, 1ull << nb, sigma, inv, nb, launch, &cfg, &wrote);
+  if (st) return st;
+  if (wrote) {
+    *done = true;
+    return QTNH_OK;
+  }
+  PermBitsArgs args;
+  int u, no;
+  if (!plan_bits(sigma, inv, nb, cfg, args, &u, &no) || !launch(args, u, no, cfg)) return QTNH_OK;
+  *done = true;
+  return check_hip(ctx, hipGetLastError(), "permute_gate launch");
+}
 }  // namespace
+
+qtnh_status launch_permute_gate(qtnh_ctx_t ctx, const c128 *A, c128 *C, const c128 *B, int nb, const int *sigma,
+                                int gate_bits, const uint32_t *xb, bool *done) {
+  *done = false;
+  if (nb < 12 || nb > 64) return QTNH_OK;
+  if (gate_bits == 1) return permute_gate<1>(ctx, A, C, B, nb, sigma, xb, done);
+  if (gate_bits == 2) return permute_gate<2>(ctx, A, C, B, nb, sigma, xb, done);
+  return QTNH_OK;
+}
 
 qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
                            const uint16_t *ptup) {

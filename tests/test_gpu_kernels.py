@@ -347,6 +347,34 @@ def test_contract_row_kernel_every_shape_bitexact(nb, kb):
     assert same_bits(host(out), ref)
 
 
+@pytest.mark.parametrize("rank,axes,flip", [
+    (16, [15, 14], False),  # the state's two lowest bits
+    (17, [0, 1], False),    # its two highest
+    (18, [3, 12], True),    # mixed, the wires given in the other order (other term order)
+    (20, [19, 0], True),    # lowest and highest
+    (19, [6, 7], False),
+    (16, [7], False),       # 1-qubit gates
+    (19, [18], False),
+    (19, [0], False),
+])
+def test_contract_dense_gate_fused_permute_bitexact(rank, axes, flip):
+    """A square dense gate (1 or 2 term bits) contracted into a state of at
+    least 2^16 elements with the default placement runs as the bit permutation
+    that moves the gate's axes last, each row of results formed in the
+    permute's tiles: equal to the oracle bit for bit."""
+    k = len(axes)
+    da, db = [2] * rank, [2] * (2 * k)
+    wires = [(a, i) for i, a in enumerate(axes)]
+    if flip:
+        wires = wires[::-1]
+    a = rnd(1 << rank, 60 + rank)
+    b = rnd(1 << (2 * k), 70 + k)
+    out, dc, _ = ops.contract(dev(a), da, dev(b), db, wires)
+    ref, rdc = oracle.contract(a, da, b, db, wires)
+    assert dc == rdc
+    assert same_bits(host(out), ref)
+
+
 def test_contract_small_gate_custom_placement_bitexact():
     """Gate outputs placed below the state's open axes too (not only last)."""
     da, db = [2] * 15, [2, 2, 2, 2]
