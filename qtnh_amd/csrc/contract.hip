@@ -541,18 +541,18 @@ qtnh_status launch_contract_direct(qtnh_ctx_t ctx, const c128 *A, int rA, const 
     bool b_low_only = nb_low >= 1;
     for (int p = nb_low; p < LC.nbits; ++p) b_low_only = b_low_only && ob.bbit[p] < 0;
     const uint64_t cap = (uint64_t)std::max(1, ctx->num_cus) * 16;
-    // A square gate of 1 or 2 term bits (a dense gate contracted into a large
+    // A square gate of 1 to 3 term bits (a dense gate contracted into a large
     // state with the default placement): the bit permutation that takes A's
     // term bits to the lowest result bits and its other bits to their result
     // bits, with each row of results formed in the permute's tiles.
     const uint64_t nA = 1ull << LA.nbits;
-    if (b_low_only && cb.kbits == nb_low && nb_low <= 2 && LC.nbits >= 16 && LA.nbits == LC.nbits &&
+    if (b_low_only && cb.kbits == nb_low && nb_low <= 3 && LC.nbits >= 16 && LA.nbits == LC.nbits &&
         (C + P.nC <= A || A + nA <= C)) {
       int sigma[64];
       for (int p = nb_low; p < LC.nbits; ++p) sigma[ob.abit[p]] = p;
       for (int c = 0; c < cb.kbits; ++c) sigma[__builtin_ctzll(cb.fa[c] ^ (c ? cb.fa[c - 1] : 0ull))] = c;
       const int N = 1 << nb_low;
-      uint32_t xb[16];
+      uint32_t xb[64];
       for (int t = 0; t < N; ++t)
         for (int o = 0; o < N; ++o) {
           uint64_t x = 0;

@@ -96,7 +96,7 @@ BitLayout bit_layout(int rank, const uint64_t *dims);
 qtnh_status launch_fill(qtnh_ctx_t ctx, c128 *buf, uint64_t n, uint64_t seed, int dist);
 qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank, const uint64_t *dims,
                            const uint16_t *ptup);
-// A square dense gate of gate_bits (1 or 2) term bits contracted into A with
+// A square dense gate of gate_bits (1 to 3) term bits contracted into A with
 // its outputs on the lowest result bits, as the bit permutation sigma (A bit
 // -> C bit, the term bits to C bits 0 .. gate_bits-1) with each result row
 // formed in the tiles (reference term order and arithmetic); xb[t 2^gb + o]:

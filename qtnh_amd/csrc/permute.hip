@@ -676,6 +676,7 @@ qtnh_status launch_permute_gate(qtnh_ctx_t ctx, const c128 *A, c128 *C, const c1
   if (nb < 12 || nb > 64) return QTNH_OK;
   if (gate_bits == 1) return permute_gate<1>(ctx, A, C, B, nb, sigma, xb, done);
   if (gate_bits == 2) return permute_gate<2>(ctx, A, C, B, nb, sigma, xb, done);
+  if (gate_bits == 3) return permute_gate<3>(ctx, A, C, B, nb, sigma, xb, done);
   return QTNH_OK;
 }
 

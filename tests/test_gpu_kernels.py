@@ -356,9 +356,12 @@ def test_contract_row_kernel_every_shape_bitexact(nb, kb):
     (16, [7], False),       # 1-qubit gates
     (19, [18], False),
     (19, [0], False),
+    (16, [15, 14, 13], False),  # 3-qubit gates
+    (18, [0, 9, 17], True),
+    (17, [2, 11, 5], False),
 ])
 def test_contract_dense_gate_fused_permute_bitexact(rank, axes, flip):
-    """A square dense gate (1 or 2 term bits) contracted into a state of at
+    """A square dense gate (1 to 3 term bits) contracted into a state of at
     least 2^16 elements with the default placement runs as the bit permutation
     that moves the gate's axes last, each row of results formed in the
     permute's tiles: equal to the oracle bit for bit."""
