@@ -110,6 +110,68 @@ __global__ void __launch_bounds__(kGThreads) grouped_direct_kernel(const GDesc *
   }
 }
 
+// The leading run of one network's steps that are small direct contractions
+// (a chain): descriptors [first, first + count), in step order.
+struct ChainDesc {
+  uint32_t first, count;
+};
+
+// Chains, one workgroup each: the workgroup runs its network's chain steps in
+// order, each element as grouped_direct_kernel computes it (same terms, same
+// order, same arithmetic), with a barrier between steps.  A step reads only
+// the network's leaves and the results of earlier steps of its chain, written
+// by the same workgroup: all of it stays on one compute unit, and the barrier
+// (which waits for the stores) orders those writes before the next step's
+// reads.  Replaces one grouped launch per step index for these steps.
+__global__ void __launch_bounds__(kGThreads) chain_direct_kernel(const GDesc *__restrict__ descs,
+                                                                 const ChainDesc *__restrict__ chains, int nchains) {
+  __shared__ uint64_t flip_a[64], flip_b[64];
+  for (int c = blockIdx.x; c < nchains; c += gridDim.x) {
+    const ChainDesc ch = chains[c];
+    for (uint32_t k = 0; k < ch.count; ++k) {
+      const GDesc &D = descs[ch.first + k];
+      __syncthreads();  // the previous step's results are stored; its mask readers are done
+      if ((int)threadIdx.x < D.kbits) {
+        uint64_t fa = 0, fb = 0;
+        for (int j = 0; j <= (int)threadIdx.x; ++j) {
+          fa |= 1ull << D.ka[j];
+          fb |= 1ull << D.kb[j];
+        }
+        flip_a[threadIdx.x] = fa;
+        flip_b[threadIdx.x] = fb;
+      }
+      __syncthreads();
+      const uint64_t K = 1ull << D.kbits;
+      for (uint64_t e = threadIdx.x; e < D.nC; e += kGThreads) {
+        uint64_t oa = 0, ob = 0;
+        for (int p = 0; p < D.ncb; ++p) {
+          if (!((e >> p) & 1ull)) continue;
+          if (D.abit[p] >= 0)
+            oa |= 1ull << D.abit[p];
+          else
+            ob |= 1ull << D.bbit[p];
+        }
+        double ar = 0.0, ai = 0.0;
+        uint64_t xa = oa, xb = ob;
+        for (uint64_t t = 0; t < K; ++t) {
+          const double2_t a = D.A[xa], b = D.B[xb];
+          double pr, pi;
+          cmul_parts(a.x, a.y, b.x, b.y, &pr, &pi);
+          ar = ar + pr;
+          ai = ai + pi;
+          if (t + 1 < K) {
+            const int cb = __builtin_ctzll(t + 1);
+            xa ^= flip_a[cb];
+            xb ^= flip_b[cb];
+          }
+        }
+        D.C[e] = double2_t{ar, ai};
+      }
+    }
+    __syncthreads();  // the next chain's masks are written after this one's readers finish
+  }
+}
+
 // Copies results (src offset, count) to a packed output region.
 struct Gather {
   const double2_t *src;
@@ -808,6 +870,17 @@ bool grouped_step(const qtnh_net_s &n, int prec, size_t s, uint64_t sum) {
   return st.groupable && step_precision(st, prec) == QTNH_PREC_FP64 && sum + st.ws64 <= kGroupWsTotal;
 }
 
+// Steps of a network's leading run of small direct contractions (operands and
+// result of at most 2^12 elements) run in its chain, one workgroup stepping
+// through them (chain_direct_kernel): measured on 256 grid-circuit networks,
+// 97% of the steps, in 217 instead of 609 step indices of grouped launches.
+constexpr uint64_t kChainMaxElems = 1ull << 12;
+bool chain_step(const qtnh_net_s &n, size_t s) {
+  const auto &st = n.steps[s];
+  return st.direct && st.binary && n.nodes[st.a].size <= kChainMaxElems && n.nodes[st.b].size <= kChainMaxElems &&
+         n.nodes[st.c].size <= kChainMaxElems;
+}
+
 // Device bytes of a wave (a consecutive run of networks), network by network:
 // leaves, arenas and results, plus a workspace for the larger of one
 // individual step and one step index's grouped steps.
@@ -858,6 +931,11 @@ struct Wave {
   std::vector<uint32_t> dk;
   std::vector<size_t> dk_off;
   std::vector<uint64_t> dfirst;
+  // Network i's steps [0, chain_len[i]) run in chain_direct_kernel (chains: one
+  // per network with a chain, descriptors at the front of the list).
+  std::vector<size_t> chain_len;
+  std::vector<ChainDesc> chains;
+  ChainDesc *dchains = nullptr;
   std::vector<Gather> gathers;
   c128 *base = nullptr, *res = nullptr;
   char *gws = nullptr;
@@ -916,9 +994,22 @@ struct Wave {
       loff += round16(n->leaf_elems);
       aoff += round16(n->arena_elems);
     }
-    // Descriptor positions, network by network (each network's steps are
-    // contiguous in memory): descriptors of a step index in network order, and
-    // each descriptor's first block from the step templates' result sizes.
+    // Descriptor positions.  First each network's chain (its leading small
+    // direct steps, chain_direct_kernel), network by network; then, for every
+    // step index, that index's other direct steps in network order, each with
+    // its first block in the index's grouped launch (from the step templates'
+    // result sizes).
+    chain_len.assign(nets.size(), 0);
+    chains.clear();
+    size_t nchain = 0;
+    for (size_t i = 0; i < nets.size(); ++i) {
+      const auto &n = *nets[i];
+      size_t c = 0;
+      while (c < n.steps.size() && chain_step(n, c)) ++c;
+      chain_len[i] = c;
+      if (c) chains.push_back({(uint32_t)nchain, (uint32_t)c});
+      nchain += c;
+    }
     starts.assign(steps + 1, 0);
     dk_off.assign(nets.size(), 0);
     size_t ndk = 0;
@@ -926,18 +1017,24 @@ struct Wave {
       const auto *n = nets[i];
       dk_off[i] = ndk;
       ndk += n->steps.size();
-      for (size_t s = 0; s < n->steps.size(); ++s) starts[s + 1] += n->steps[s].direct && n->steps[s].binary;
+      for (size_t s = chain_len[i]; s < n->steps.size(); ++s) starts[s + 1] += n->steps[s].direct && n->steps[s].binary;
     }
+    starts[0] = nchain;
     for (size_t s = 0; s < steps; ++s) starts[s + 1] += starts[s];
     ndesc = starts[steps];
     dk.assign(ndk, 0);
     dfirst.assign(ndesc, 0);
     {
       std::vector<size_t> next(starts.begin(), starts.end() - 1);
+      size_t kc = 0;
       for (size_t i = 0; i < nets.size(); ++i) {
         const auto *n = nets[i];
         for (size_t s = 0; s < n->steps.size(); ++s) {
           if (!n->steps[s].direct || !n->steps[s].binary) continue;
+          if (s < chain_len[i]) {
+            dk[dk_off[i] + s] = (uint32_t)kc++;  // first block unused (dfirst 0)
+            continue;
+          }
           const size_t k = next[s]++;
           dk[dk_off[i] + s] = (uint32_t)k;
           dfirst[k] = gdesc_blocks(n->gdesc[s]);  // block count; prefix sums below
@@ -1084,8 +1181,8 @@ struct Wave {
     }
   }
 
-  // Descriptors of step indices [s0, s1) into out[starts[s0] .. starts[s1]).
-  // Every direct-step descriptor, network by network, networks split over up
+  // Every direct-step descriptor (chains and step indices) into out[0 .. ndesc),
+  // network by network, networks split over up
   // to kPlanThreads host threads (measured: 8-13 ms for the 93k descriptors
   // of a 256-network batch when built step index by step index on one thread,
   // which touches every network per step index).
@@ -1113,6 +1210,7 @@ struct Wave {
 
   uint64_t desc_bytes() const {
     return pad256(ndesc * sizeof(GDesc)) + pad256(gathers.size() * sizeof(Gather)) +
+           pad256(chains.size() * sizeof(ChainDesc)) +
            pad256(pin.size() * sizeof(PermDesc)) + pad256(pout.size() * sizeof(PermDesc)) +
            pad256(zg[0].size() * sizeof(ZGroupDesc)) + pad256(zg[1].size() * sizeof(ZGroupDesc)) +
            pad256(zsum.size() * sizeof(ZSumDesc)) + pad256(sk.size() * sizeof(SkDesc)) + 256;
@@ -1124,6 +1222,8 @@ struct Wave {
     p += pad256(ndesc * sizeof(GDesc));
     dgath = (Gather *)p;
     p += pad256(gathers.size() * sizeof(Gather));
+    dchains = (ChainDesc *)p;
+    p += pad256(chains.size() * sizeof(ChainDesc));
     d_pin = (PermDesc *)p;
     p += pad256(pin.size() * sizeof(PermDesc));
     d_pout = (PermDesc *)p;
@@ -1152,6 +1252,7 @@ struct Wave {
       if (bytes) memcpy(staging + ((const char *)dst - d0), src, bytes);
     };
     put(dgath, gathers.data(), gathers.size() * sizeof(Gather));
+    put(dchains, chains.data(), chains.size() * sizeof(ChainDesc));
     put(d_pin, pin.data(), pin.size() * sizeof(PermDesc));
     put(d_pout, pout.data(), pout.size() * sizeof(PermDesc));
     put(d_zg[0], zg[0].data(), zg[0].size() * sizeof(ZGroupDesc));
@@ -1183,6 +1284,7 @@ struct Wave {
         st = check_hip(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream),
                        "batch grouped descriptor upload");
     };
+    put(dchains, chains.data(), chains.size() * sizeof(ChainDesc));
     put(d_pin, pin.data(), pin.size() * sizeof(PermDesc));
     put(d_pout, pout.data(), pout.size() * sizeof(PermDesc));
     put(d_zg[0], zg[0].data(), zg[0].size() * sizeof(ZGroupDesc));
@@ -1218,8 +1320,14 @@ struct Wave {
     return enqueue_gather(ctx);
   }
 
-  // The kernels of step indices [s0, s1).
+  // The kernels of step indices [s0, s1) (from index 0: the chains first).
   qtnh_status enqueue_steps(qtnh_ctx_t ctx, size_t s0, size_t s1) {
+    if (s0 == 0 && !chains.empty()) {
+      hipLaunchKernelGGL(chain_direct_kernel, dim3((unsigned)chains.size()), dim3(kGThreads), 0, ctx->stream, ddesc,
+                         dchains, (int)chains.size());
+      qtnh_status st = check_hip(ctx, hipGetLastError(), "chain launch");
+      if (st) return st;
+    }
     for (size_t s = s0; s < s1; ++s) {
       const size_t b = starts[s], e = starts[s + 1];
       if (e > b) {
