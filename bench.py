@@ -394,6 +394,13 @@ def bench_c3_gemm(torch, ops, _lib, n, seed):
     out["split_speedup"] = round(out["fp64"]["ms"] / out["split"]["ms"], 2)
     out["split_rel_err"] = float(((cs - c64).norm() / c64.norm()).item())
     out["fp64_frac_of_peak"] = round(out["fp64"]["TFLOP/s"] / 78.6, 3)
+    # The split path's matrix-core work: three real products (Gauss), each the
+    # S (S + 1) / 2 = 15 digit products of 5 slices, 2 n^3 int8 ops apiece, over
+    # the whole contraction's time (operand splitting included), against the
+    # 5.0 POP/s dense int8 peak.
+    int8_ops = 3 * 15 * 2.0 * n ** 3
+    out["split"]["int8_POP/s"] = round(int8_ops / out["split"]["ms"] / 1e12, 3)
+    out["split"]["int8_frac_of_peak"] = round(out["split"]["int8_POP/s"] / 5.0, 3)
     del a, b, c64, cs
     torch.cuda.empty_cache()
     return out
