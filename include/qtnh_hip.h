@@ -143,9 +143,11 @@ qtnh_status qtnh_contract_c128(qtnh_ctx_t ctx, const void *A, int rA, const uint
  * planes, split-K and depth-chunk partials) is a file-scope device array of
  * QTNH_NOALLOC_SCRATCH_BYTES that every GPU holds until the process ends, its
  * address looked up once per device by qtnh_ctx_create.  A contraction whose
- * workspace exceeds it returns QTNH_EINVAL before any launch.  Calls that
- * overlap in time on one GPU (other streams, other threads) share the array
- * and race; calls ordered on one stream do not. */
+ * workspace exceeds it returns QTNH_EINVAL before any launch.  Kernels that
+ * time their tile shapes on first use keep the cached or the default shape
+ * during the call (no events, no synchronisation; bits do not depend on the
+ * shape).  Calls that overlap in time on one GPU (other streams, other
+ * threads) share the array and race; calls ordered on one stream do not. */
 qtnh_status qtnh_contract_c128_noalloc(qtnh_ctx_t ctx, const void *A, int rA, const uint64_t *dA, const void *B,
                                        int rB, const uint64_t *dB, int nw, const uint16_t *wires,
                                        const uint16_t *repl1, const uint16_t *repl2, void *C, int precision,

@@ -37,6 +37,10 @@ struct qtnh_ctx_s {
   // Fused-circuit tile shape chosen by the autotuner, per qubit count (0: unset).
   int8_t circ_tb[64] = {0};
   int8_t circ_low[64] = {0};
+  // Set for the duration of a call that must create no HIP objects and not
+  // synchronise (qtnh_contract_c128_noalloc): kernels that time their tile
+  // shapes on first use take the cached or the default shape instead.
+  bool fixed_shapes = false;
 };
 
 namespace qtnh_hip {

@@ -23,9 +23,12 @@
 namespace {
 
 constexpr uint64_t kScratchBytes = QTNH_NOALLOC_SCRATCH_BYTES;
-// Permutes inside a contraction move at most kScratchBytes / 16 elements
-// through the array; permutes of fewer than 2^23 elements never time tile
-// shapes, so no call here creates events or touches the permute tuner.
+// The call runs with the context's fixed_shapes set: kernels that time their
+// tile shapes on first use (the permute, and the dense-gate contraction that
+// runs as a permute) keep the cached or the default shape, so no call here
+// creates events or synchronises.  (Permutes inside a contraction move at most
+// kScratchBytes / 16 elements through the array, under the tuner's size
+// anyway; a dense gate contracted into a large state needs no workspace.)
 static_assert(kScratchBytes / 16 <= (1ull << 22), "scratch permutes must stay below the permute tuner's size");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -81,6 +84,9 @@ qtnh_status qtnh_contract_c128_noalloc(qtnh_ctx_t ctx, const void *A, int rA, co
   }
   if (!ws) return qtnh_hip::set_error(ctx, QTNH_EHIP, "contract_noalloc: scratch address of device %d not set up",
                                       ctx->device);
-  return qtnh_hip::launch_contract(ctx, (const c128 *)A, rA, dA, (const c128 *)B, rB, dB, nw, wires, repl1, repl2,
-                                   (c128 *)C, precision, err_out, ws, kScratchBytes);
+  ctx->fixed_shapes = true;
+  st = qtnh_hip::launch_contract(ctx, (const c128 *)A, rA, dA, (const c128 *)B, rB, dB, nw, wires, repl1, repl2,
+                                 (c128 *)C, precision, err_out, ws, kScratchBytes);
+  ctx->fixed_shapes = false;
+  return st;
 }

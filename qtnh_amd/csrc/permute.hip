@@ -597,6 +597,7 @@ qtnh_status tuned_shape(qtnh_ctx_t ctx, const std::string &key, uint64_t n, cons
       return QTNH_OK;
     }
   }
+  if (ctx->fixed_shapes) return QTNH_OK;  // no timing (events, synchronisation) in this call
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess) {
     (void)hipGetLastError();  // not tuned: the query's error is not the caller's
@@ -714,6 +715,7 @@ qtnh_status launch_permute(qtnh_ctx_t ctx, const c128 *src, c128 *dst, int rank,
         }
       }
       hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+      if (!found && ctx->fixed_shapes) found = true;  // keep the default shape: no timing in this call
       hipError_t qe = found ? hipSuccess : hipStreamIsCapturing(ctx->stream, &cap);
       if (qe != hipSuccess) (void)hipGetLastError();  // not tuned: the query's error is not the caller's
       if (!found && qe == hipSuccess && cap == hipStreamCaptureStatusNone) {

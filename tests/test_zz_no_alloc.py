@@ -93,6 +93,28 @@ def test_noalloc_equals_contract(route, precision, first, second):
         assert eta_got == eta_want
 
 
+def test_noalloc_dense_gate_on_a_large_state():
+    """A 2-qubit dense gate contracted into a rank-24 state needs no workspace
+    and runs as the fused permute.  Called first through the allocation-free
+    entry point (a permutation no call has tuned yet), it keeps the default
+    tile shape without timing shapes; the regular entry point afterwards may
+    time them.  Tile shapes do not change bits: both agree, on two calls in a
+    row (the second smaller)."""
+    for call_no, mb in enumerate((22, 20)):
+        print("noalloc dense gate call %d: rank %d" % (call_no, mb + 2), flush=True)
+        A, dA, B, dB, wires, nC = case(900 + mb, mb, 2, 2)
+        got = ops.empty(nC)
+        want = ops.empty(nC)
+        got.fill_(float("nan"))
+        st, _ = call("qtnh_contract_c128_noalloc", A, dA, B, dB, wires, _lib.PREC_FP64, got)
+        assert st == 0, _lib.lib().qtnh_ctx_last_error(ops.context())
+        st, _ = call("qtnh_contract_c128", A, dA, B, dB, wires, _lib.PREC_FP64, want)
+        assert st == 0, _lib.lib().qtnh_ctx_last_error(ops.context())
+        assert same_bits(got, want), call_no
+        del A, B, got, want
+        torch.cuda.empty_cache()
+
+
 def test_noalloc_refuses_a_workspace_beyond_the_scratch():
     """M = N = K = 1024 with permuted operands needs 16 MiB per operand copy,
     16 MiB for the natural result and 64 MiB of depth-chunk partials: more
