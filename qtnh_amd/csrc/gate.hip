@@ -398,10 +398,11 @@ qtnh_status launch_gate_low3(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *m
     }
     args.gbase[q] = (uint16_t)off;
   }
-  // One tile per wave: a grid of resident waves looping over tiles with the
-  // next tile's loads in flight measured slower (5.03 against 5.27 TB/s).
+  // One tile per wave (64 threads per tile), up to the streaming grid limit.
+  // Rank 28: 5.36 TB/s; a grid of 512 workgroups (8 waves per CU) looping
+  // over tiles gives 5.09, and with the next tile's loads in flight 5.03.
   const uint64_t n_tiles = n >> 9;
-  const unsigned blocks = grid_for((n_tiles + 3) / 4, kGateThreads, kMaxStreamBlocks);
+  const unsigned blocks = grid_for(n_tiles * 64, kGateThreads, g_stream_blocks);
   hipLaunchKernelGGL(gate_low3_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, n_tiles);
   return check_hip(ctx, hipGetLastError(), "gate launch");
 }
