@@ -313,12 +313,15 @@ __global__ void __launch_bounds__(kGateThreads) gate_lanes_kernel(c128 *__restri
 // coalesced loads, each lane then takes one of the tile's 64 groups of 8
 // elements and applies the gate as gate_bits_kernel does (same arithmetic,
 // same order), and the results leave through LDS with coalesced stores.  The
+// coefficients are wave-uniform: each output's 8 come from the argument block
+// by scalar loads (an LDS copy read per use measured 1.5% slower: its 64
+// 16-byte broadcast reads per tile and lane doubled the kernel's LDS traffic).  The
 // LDS slot of tile element e is e ^ ((e / 8) % 8): conflict-free for the row
 // accesses and for group reads when the gate bits are the lowest three (at
 // most 2-way otherwise).  Waves synchronise only with themselves (their
 // region is private), so every wave loops over its own tiles.
 struct Low3Args {
-  c128 g[64];        // G[i][o], row-major
+  c128 gt[64];       // G[i][o] at gt[8 * o + i]: each output's coefficients contiguous
   uint16_t coff[8];  // tile offset of gate combination c (MSB-first gate bits)
   uint16_t gbase[64];  // tile offset of group q: its bits spread over the non-gate tile bits
 };
@@ -328,24 +331,31 @@ __device__ inline uint32_t low3_slot(uint32_t e) { return e ^ ((e >> 3) & 7u); }
 __global__ void __launch_bounds__(kGateThreads) gate_low3_kernel(c128 *__restrict__ psi, Low3Args args,
                                                                  uint64_t n_tiles) {
   constexpr int kWaves = kGateThreads / 64;
-  __shared__ double2_t gl[64];
   __shared__ double2_t buf[kWaves][512];
-  for (int i = threadIdx.x; i < 64; i += kGateThreads) gl[i] = double2_t{args.g[i].re, args.g[i].im};
-  __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double2_t *p2 = reinterpret_cast<double2_t *>(psi);
   double2_t *b = buf[w];
   const uint32_t gb = args.gbase[lane];
-  for (uint64_t t = (uint64_t)blockIdx.x * kWaves + w; t < n_tiles; t += (uint64_t)gridDim.x * kWaves) {
-    double2_t *src = p2 + (t << 9);
-    double2_t v[8];
+  const uint64_t step = (uint64_t)gridDim.x * kWaves;
+  uint64_t t = (uint64_t)blockIdx.x * kWaves + w;
+  if (t >= n_tiles) return;  // per wave: each wave synchronises only with itself
+  double2_t v[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = __builtin_nontemporal_load(src + 64 * j + lane);
+  for (int j = 0; j < 8; ++j) v[j] = __builtin_nontemporal_load(p2 + (t << 9) + 64 * j + lane);
+  for (;;) {
+    double2_t *src = p2 + (t << 9);
 #pragma unroll
     for (int j = 0; j < 8; ++j) b[low3_slot(64 * j + lane)] = v[j];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // The wave's next tile is loaded while this one is formed and stored.
+    const uint64_t tn = t + step;
+    const bool more = tn < n_tiles;  // wave-uniform
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = __builtin_nontemporal_load(p2 + (tn << 9) + 64 * j + lane);
+    }
     double2_t in[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) in[c] = b[low3_slot(gb + args.coff[c])];
@@ -353,11 +363,16 @@ __global__ void __launch_bounds__(kGateThreads) gate_low3_kernel(c128 *__restric
     // reads: each output is stored as soon as it is formed.
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
+      // The output index is hidden from the optimiser so that each output's
+      // 8 coefficients are loaded (scalar loads) where they are used rather
+      // than all 64 hoisted out of the loop.
+      int oc = o;
+      asm volatile("" : "+s"(oc));
+      const c128 *col = args.gt + 8 * oc;
       double ar = 0.0, ai = 0.0;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const double2_t gg = gl[i * 8 + o];
-        const c128 pr = cmul(c128{in[i].x, in[i].y}, c128{gg.x, gg.y});
+        const c128 pr = cmul(c128{in[i].x, in[i].y}, col[i]);
         ar = ar + pr.re;
         ai = ai + pr.im;
       }
@@ -368,17 +383,20 @@ __global__ void __launch_bounds__(kGateThreads) gate_low3_kernel(c128 *__restric
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int j = 0; j < 8; ++j) __builtin_nontemporal_store(b[low3_slot(64 * j + lane)], src + 64 * j + lane);
+    if (!more) break;
     // The next tile's stores into b come after these reads.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    t = tn;
   }
 }
 
 qtnh_status launch_gate_low3(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *msb_bits, const double *G) {
   Low3Args args;
   memset(&args, 0, sizeof(args));
-  memcpy(args.g, G, sizeof(c128) * 64);
+  for (int i = 0; i < 8; ++i)
+    for (int o = 0; o < 8; ++o) memcpy(&args.gt[8 * o + i], G + 2 * (8 * i + o), sizeof(c128));
   uint32_t gmask = 0;
   for (int j = 0; j < 3; ++j) gmask |= 1u << msb_bits[j];
   for (int c = 0; c < 8; ++c) {
@@ -399,8 +417,9 @@ qtnh_status launch_gate_low3(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *m
     args.gbase[q] = (uint16_t)off;
   }
   // One tile per wave (64 threads per tile), up to the streaming grid limit.
-  // Rank 28: 5.36 TB/s; a grid of 512 workgroups (8 waves per CU) looping
-  // over tiles gives 5.09, and with the next tile's loads in flight 5.03.
+  // Rank 28: 5.52 TB/s.  Fewer waves each looping over tiles, with the next
+  // tile's loads in flight, are slower: 5.38 at two tiles per wave, 5.29 at
+  // sixteen, 5.09 (before the scalar coefficient loads) at 512 workgroups.
   const uint64_t n_tiles = n >> 9;
   const unsigned blocks = grid_for(n_tiles * 64, kGateThreads, g_stream_blocks);
   hipLaunchKernelGGL(gate_low3_kernel, dim3(blocks), dim3(kGateThreads), 0, ctx->stream, psi, args, n_tiles);

@@ -204,10 +204,12 @@ def test_swap_rejects_unequal_dims():
 
 # ------------------------------------------------------------------ gates
 # Placements that reach every gate kernel shape: all gate bits among the lowest
-# 6 (lane exchange, registers), some low and some high (lane exchange, LDS
-# coefficients, one or two high bits), all high (one thread per group).
+# 6 (lane exchange, registers; 3-qubit: LDS-staged tiles, in any bit order),
+# some low and some high (lane exchange, LDS coefficients, one or two high
+# bits), all high (one thread per group).
 _GATE_PLACEMENTS = {1: [[13], [0]], 2: [[13, 12], [13, 0], [0, 13], [1, 2]],
-                    3: [[13, 12, 11], [13, 12, 0], [13, 0, 5], [0, 13, 11], [0, 1, 2], [7, 13, 3]]}
+                    3: [[13, 12, 11], [13, 12, 0], [13, 0, 5], [0, 13, 11], [0, 1, 2], [7, 13, 3],
+                        [13, 9, 11], [12, 13, 8]]}
 
 
 @pytest.mark.parametrize("k", [1, 2, 3])
