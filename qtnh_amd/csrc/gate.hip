@@ -37,7 +37,7 @@ __device__ inline c128 cmul(c128 a, c128 b) {
 
 template <int KB>
 struct GateArgs {
-  c128 g[(1 << KB) * (1 << KB)];  // G[i][o], row-major
+  c128 g[(1 << KB) * (1 << KB)];  // G[i][o] at g[i * 2^KB + o]; 3-qubit gates: at g[8 * o + i]
   uint64_t comb[1 << KB];         // offset of gate combination c from the group base
   int zpos[KB];                   // gate bit positions, ascending (zeros inserted here)
 };
@@ -50,27 +50,20 @@ typedef double double2_t __attribute__((ext_vector_type(2)));
 // cover whole 128-byte lines (with lower gate bits the lanes of one load are
 // strided inside lines, and the next load of the same lines must hit in cache).
 //
-// 3-qubit gates read their 64 coefficients from an LDS copy (made once per
-// workgroup) at each use: the 128 doubles do not fit the scalar registers, and
-// reloading them from the argument block per use is slower (measured 5.25 ->
-// 5.57 TB/s at 2^28 elements).  The empty asm with a memory clobber at the top
-// of each group keeps the compiler from hoisting the loop-invariant LDS reads
-// into registers (it did: 320 VGPRs).
+// 3-qubit gates: the 128 doubles of the coefficients do not fit the scalar
+// registers, so each output's 8 (one column of G, contiguous in the argument
+// block) are loaded by scalar loads where they are used; the output index is
+// hidden from the optimiser, which otherwise hoists all 64 out of the loop and
+// spills them into vector registers.  An LDS copy read per use measured
+// slower.
 template <int KB, bool NT>
 __global__ void __launch_bounds__(kGateThreads) gate_bits_kernel(c128 *__restrict__ psi, GateArgs<KB> args,
                                                                  uint64_t n_groups) {
   constexpr int K = 1 << KB;
   constexpr int U = KB <= 2 ? 2 : 1;
-  constexpr bool kLdsCoef = KB == 3;
-  __shared__ double2_t gl[kLdsCoef ? K * K : 1];
-  if constexpr (kLdsCoef) {
-    for (int i = threadIdx.x; i < K * K; i += kGateThreads) gl[i] = double2_t{args.g[i].re, args.g[i].im};
-    __syncthreads();
-  }
   double2_t *p2 = reinterpret_cast<double2_t *>(psi);
   const uint64_t stride = (uint64_t)gridDim.x * kGateThreads;
   for (uint64_t g0 = (uint64_t)blockIdx.x * kGateThreads + threadIdx.x; g0 < n_groups; g0 += stride * U) {
-    if constexpr (kLdsCoef) asm volatile("" ::: "memory");
     uint64_t base[U];
     double2_t in[U][K];
 #pragma unroll
@@ -91,11 +84,13 @@ __global__ void __launch_bounds__(kGateThreads) gate_bits_kernel(c128 *__restric
       if (g0 + u * stride >= n_groups) continue;
 #pragma unroll
       for (int o = 0; o < K; ++o) {
+        int oc = o;
+        if (KB == 3) asm volatile("" : "+s"(oc));
         double ar = 0.0, ai = 0.0;
 #pragma unroll
         for (int i = 0; i < K; ++i) {
           const c128 x = {in[u][i].x, in[u][i].y};
-          const c128 gg = kLdsCoef ? c128{gl[i * K + o].x, gl[i * K + o].y} : args.g[i * K + o];
+          const c128 gg = KB == 3 ? args.g[K * oc + i] : args.g[i * K + o];
           const c128 pr = cmul(x, gg);
           ar = ar + pr.re;
           ai = ai + pr.im;
@@ -484,7 +479,8 @@ qtnh_status launch_gate_bits(qtnh_ctx_t ctx, c128 *psi, uint64_t n, const int *m
   constexpr int K = 1 << KB;
   GateArgs<KB> args;
   memset(&args, 0, sizeof(args));
-  memcpy(args.g, G, sizeof(c128) * K * K);
+  for (int i = 0; i < K; ++i)
+    for (int o = 0; o < K; ++o) memcpy(&args.g[KB == 3 ? K * o + i : K * i + o], G + 2 * (K * i + o), sizeof(c128));
   for (int c = 0; c < K; ++c) {
     uint64_t off = 0;
     for (int j = 0; j < KB; ++j)
