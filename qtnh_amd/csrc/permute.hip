@@ -13,6 +13,7 @@
 //
 // Other dims: one thread per result element with mixed-radix decoding.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -320,10 +321,13 @@ __global__ void swap_general_kernel(c128 *__restrict__ buf, uint64_t n, uint64_t
 
 // Tunables of the bit-permute tile.  a / b: starting source-low / result-low
 // bits; the tile then grows until it holds u_target bits (<= 12, 64 KiB of
-// LDS).  order 0 walks tiles by source bit position, 1 by the larger of the
-// source and result positions (page locality on both sides).  nt: loads and
-// stores bypass the caches.  blocks_per_cu: blocks per compute unit, each
-// looping over tiles.
+// LDS).  order bit 0: walk tiles by source bit position (0) or by the larger
+// of the source and result positions (1, page locality on both sides).
+// order bit 1: fill the tile up to u_target with outer bits whose source and
+// result positions lie near 2^10 elements (runs of 2^a and 2^b elements at
+// strides of 16 KiB or so), instead of growing a and b (longer runs).  nt:
+// loads and stores bypass the caches.  blocks_per_cu: blocks per compute unit,
+// each looping over tiles.
 struct PermTune {
   int a = 6, b = 6, u_target = 12, order = 1, nt = 1, blocks_per_cu = 2;
 };
@@ -332,8 +336,9 @@ static PermTune g_perm;
 // Per-permutation tuning.  No one tile shape is best for every permutation on
 // MI355X (rank-28, TB/s: the default 12-bit tile with two blocks per CU gives
 // 4.5-5.3; an 11-bit tile with 64-128 blocks per CU, each looping over a few
-// tiles, gives 5.9-6.0 on some permutations and 4.7 on others), and every
-// shape writes the same bits.  So the first permute of a
+// tiles, gives 5.9-6.0 on some permutations and 4.7 on others; where the
+// natural tile is short, filling it with outer bits near 2^10 gives 6.0-6.2
+// against 5.4-5.7 for longer runs), and every shape writes the same bits.  So the first permute of a
 // large state (>= 2^23 elements) with a new (bit count, bit permutation)
 // signature on a device times each candidate below on the caller's buffers
 // and keeps the fastest for that signature.  The timing synchronises the
@@ -341,7 +346,8 @@ static PermTune g_perm;
 // is being captured into a graph.
 static bool g_perm_auto = true;
 static const PermTune kPermCandidates[] = {{6, 6, 12, 1, 1, 2},  {6, 6, 12, 1, 1, 128}, {5, 6, 11, 0, 1, 64},
-                                           {5, 6, 11, 1, 1, 128}, {5, 6, 11, 1, 1, 32},  {6, 5, 11, 0, 1, 256}};
+                                           {5, 6, 11, 1, 1, 128}, {5, 6, 11, 1, 1, 32},  {6, 5, 11, 0, 1, 256},
+                                           {6, 6, 12, 2, 1, 128}, {6, 6, 11, 3, 1, 64},  {6, 6, 12, 3, 1, 2}};
 constexpr int kPermTimedRuns = 3;
 constexpr uint64_t kPermAutoMin = 1ull << 23;
 constexpr size_t kPermCacheMax = 4096;
@@ -416,18 +422,35 @@ bool plan_bits(const int *sigma, const int *inv, int nb, const PermTune &cfg, Pe
   memset(&args, 0, sizeof(args));
   int a = std::min(cfg.a, nb), b = std::min(cfg.b, nb);
   if (!tile_ok(inv, a, b, nb)) a = b = std::min(6, nb);
+  const bool pad = (cfg.order & 2) != 0;
   // Grow the tile while it is smaller than the target (low bits kept in place,
   // or a low result bit taken from a high source bit, give small tiles).
-  while (tile_bits(inv, a, b) < cfg.u_target) {
+  while (!pad && tile_bits(inv, a, b) < cfg.u_target) {
     int na = std::min(a + 1, nb), nbb = std::min(b + 1, nb);
     if ((na == a && nbb == b) || !tile_ok(inv, na, nbb, nb)) break;
     a = na;
     b = nbb;
   }
-  // Tile bits: source bits [0, a) then the source bits of result bits [0, b) not already in.
+  // Tile bits: source bits [0, a), the source bits of result bits [0, b) not
+  // already in, then (pad) outer bits by distance of their source and result
+  // positions from 10.
   int extra[6], ne = 0;
   for (int p = 0; p < b; ++p)
     if (inv[p] >= a) extra[ne++] = inv[p];  // ordered by result position
+  if (pad) {
+    bool in_tile[64] = {};
+    for (int p = 0; p < a; ++p) in_tile[p] = true;
+    for (int m = 0; m < ne; ++m) in_tile[extra[m]] = true;
+    auto dist = [&](int q) { return std::abs(q - 10) + std::abs(sigma[q] - 10); };
+    while (a + ne < std::min(cfg.u_target, nb) && ne < 6 && a + ne - b < 6) {
+      int best = -1;
+      for (int q = 0; q < nb; ++q)
+        if (!in_tile[q] && (best < 0 || dist(q) < dist(best))) best = q;
+      if (best < 0) break;
+      in_tile[best] = true;
+      extra[ne++] = best;
+    }
+  }
   const int u = a + ne;
   // t position of each tile source bit
   int tpos[64];
@@ -441,10 +464,11 @@ bool plan_bits(const int *sigma, const int *inv, int nb, const PermTune &cfg, Pe
     args.in_hi[j] = off;
   }
   // Result-order index s: bits [0, b) = result bits [0, b); above them the
-  // remaining tile bits (source bits in [0, a) whose result bit >= b), by result position.
+  // remaining tile bits (source bits in [0, a) or padding whose result bit
+  // >= b), by result position.
   int rest[6], nr = 0;
   for (int p = b; p < nb; ++p)
-    if (inv[p] < a) {
+    if (tpos[inv[p]] >= 0) {
       if (nr == 6) return false;
       rest[nr++] = p;  // result positions
     }
@@ -472,7 +496,7 @@ bool plan_bits(const int *sigma, const int *inv, int nb, const PermTune &cfg, Pe
   int outer[64], no = 0;
   for (int p = 0; p < nb; ++p)
     if (tpos[p] < 0) outer[no++] = p;
-  if (cfg.order == 1)
+  if (cfg.order & 1)
     std::stable_sort(outer, outer + no,
                      [&](int x, int y) { return std::max(x, sigma[x]) < std::max(y, sigma[y]); });
   for (int j = 0; j < no; ++j) {
@@ -529,6 +553,9 @@ qtnh_status tune_shapes(qtnh_ctx_t ctx, const int *sigma, const int *inv, int nb
   auto time_shape = [&](const PermTune &cand) -> float {
     PermBitsArgs args;
     int u, no;
+    // A filled tile differs from a grown one only when the natural tile is short.
+    if ((cand.order & 2) && tile_bits(inv, std::min(cand.a, nb), std::min(cand.b, nb)) >= std::min(cand.u_target, nb))
+      return -1.0f;
     if (err != hipSuccess || !plan_bits(sigma, inv, nb, cand, args, &u, &no)) return -1.0f;
     if (!launch(args, u, no, cand)) return -1.0f;  // warm-up; writes the result
     err = hipEventRecord(e0, ctx->stream);

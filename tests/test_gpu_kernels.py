@@ -94,18 +94,26 @@ def _perm_cases():
 
 
 def test_permute_every_tile_shape():
-    """Results do not depend on the tile parameters."""
+    """Results do not depend on the tile parameters: a random permutation, and
+    ones whose natural tile is small (a rotation, the low 8 axes kept), where
+    the tile grows or is filled with outer bits."""
     dims = [2] * 16
     rng = random.Random(16)
     p = list(range(16))
     rng.shuffle(p)
+    hi = list(range(8))
+    rng.shuffle(hi)
     x = rnd(1 << 16, 16)
-    ref = oracle.permute(x, dims, p)
+    perms = [p, [(i + 1) % 16 for i in range(16)], hi + list(range(8, 16))]
+    # order + 2: the tile is filled with outer bits instead of longer runs
+    shapes = [(6, 6, 12, 0, 0, 16), (4, 4, 8, 1, 1, 2), (6, 3, 9, 1, 0, 4), (3, 6, 12, 0, 1, 8), (2, 2, 4, 0, 0, 1),
+              (8, 8, 12, 1, 1, 16), (6, 6, 12, 3, 1, 2), (5, 6, 11, 2, 0, 8), (4, 6, 12, 3, 1, 4), (6, 6, 10, 3, 1, 16)]
     try:
-        for a, b, u, order, nt, bpc in [(6, 6, 12, 0, 0, 16), (4, 4, 8, 1, 1, 2), (6, 3, 9, 1, 0, 4),
-                                        (3, 6, 12, 0, 1, 8), (2, 2, 4, 0, 0, 1), (8, 8, 12, 1, 1, 16)]:
-            ops.tune_permute(a, b, u, order, nt, bpc)
-            assert same_bits(host(ops.permute(dev(x), dims, p)), ref), (a, b, u, order, nt, bpc)
+        for p in perms:
+            ref = oracle.permute(x, dims, p)
+            for a, b, u, order, nt, bpc in shapes:
+                ops.tune_permute(a, b, u, order, nt, bpc)
+                assert same_bits(host(ops.permute(dev(x), dims, p)), ref), (p, a, b, u, order, nt, bpc)
     finally:
         ops.tune_permute_auto(True)
 
