@@ -347,7 +347,8 @@ static PermTune g_perm;
 static bool g_perm_auto = true;
 static const PermTune kPermCandidates[] = {{6, 6, 12, 1, 1, 2},  {6, 6, 12, 1, 1, 128}, {5, 6, 11, 0, 1, 64},
                                            {5, 6, 11, 1, 1, 128}, {5, 6, 11, 1, 1, 32},  {6, 5, 11, 0, 1, 256},
-                                           {6, 6, 12, 2, 1, 128}, {6, 6, 11, 3, 1, 64},  {6, 6, 12, 3, 1, 2}};
+                                           {6, 6, 12, 2, 1, 128}, {6, 6, 11, 3, 1, 64},  {6, 6, 12, 3, 1, 2},
+                                           {5, 5, 11, 2, 1, 128}, {6, 5, 11, 2, 1, 128}};
 constexpr int kPermTimedRuns = 3;
 constexpr uint64_t kPermAutoMin = 1ull << 23;
 constexpr size_t kPermCacheMax = 4096;
