@@ -381,7 +381,8 @@ qtnh_status qtnh_expect_z_c128(qtnh_ctx_t ctx, const void *state, int n_qubits, 
 /* Process-wide tile parameters of the power-of-two permute kernel: starting
  * source-low / result-low tile bits a, b (1..12), target tile bits (<= 12),
  * tile walk order (0: source order, 1: page-local on both sides; + 2: a tile
- * short of the target is filled with outer bits instead of longer runs),
+ * short of the target is filled with outer bits instead of longer runs; + 4:
+ * tiles walked by their outer bits' distance from bit 10),
  * non-temporal loads and stores (0/1), persistent blocks per compute unit.  Results do not
  * depend on them.  Setting them turns per-permutation tuning off. */
 void qtnh_tune_permute(int a, int b, int u_target, int order, int nt, int blocks_per_cu);

@@ -325,7 +325,9 @@ __global__ void swap_general_kernel(c128 *__restrict__ buf, uint64_t n, uint64_t
 // of the source and result positions (1, page locality on both sides).
 // order bit 1: fill the tile up to u_target with outer bits whose source and
 // result positions lie near 2^10 elements (runs of 2^a and 2^b elements at
-// strides of 16 KiB or so), instead of growing a and b (longer runs).  nt:
+// strides of 16 KiB or so), instead of growing a and b (longer runs).
+// order bit 2: walk tiles by the distance of the outer bits' source and
+// result positions from 10 (tiles in flight together differ there).  nt:
 // loads and stores bypass the caches.  blocks_per_cu: blocks per compute unit,
 // each looping over tiles.
 struct PermTune {
@@ -348,7 +350,8 @@ static bool g_perm_auto = true;
 static const PermTune kPermCandidates[] = {{6, 6, 12, 1, 1, 2},  {6, 6, 12, 1, 1, 128}, {5, 6, 11, 0, 1, 64},
                                            {5, 6, 11, 1, 1, 128}, {5, 6, 11, 1, 1, 32},  {6, 5, 11, 0, 1, 256},
                                            {6, 6, 12, 2, 1, 128}, {6, 6, 11, 3, 1, 64},  {6, 6, 12, 3, 1, 2},
-                                           {5, 5, 11, 2, 1, 128}, {6, 5, 11, 2, 1, 128}};
+                                           {5, 5, 11, 2, 1, 128}, {6, 5, 11, 2, 1, 128}, {6, 6, 12, 4, 1, 128},
+                                           {5, 5, 11, 6, 1, 128}};
 constexpr int kPermTimedRuns = 3;
 constexpr uint64_t kPermAutoMin = 1ull << 23;
 constexpr size_t kPermCacheMax = 4096;
@@ -497,7 +500,11 @@ bool plan_bits(const int *sigma, const int *inv, int nb, const PermTune &cfg, Pe
   int outer[64], no = 0;
   for (int p = 0; p < nb; ++p)
     if (tpos[p] < 0) outer[no++] = p;
-  if (cfg.order & 1)
+  if (cfg.order & 4)
+    std::stable_sort(outer, outer + no, [&](int x, int y) {
+      return std::abs(x - 10) + std::abs(sigma[x] - 10) < std::abs(y - 10) + std::abs(sigma[y] - 10);
+    });
+  else if (cfg.order & 1)
     std::stable_sort(outer, outer + no,
                      [&](int x, int y) { return std::max(x, sigma[x]) < std::max(y, sigma[y]); });
   for (int j = 0; j < no; ++j) {

@@ -105,9 +105,11 @@ def test_permute_every_tile_shape():
     rng.shuffle(hi)
     x = rnd(1 << 16, 16)
     perms = [p, [(i + 1) % 16 for i in range(16)], hi + list(range(8, 16))]
-    # order + 2: the tile is filled with outer bits instead of longer runs
+    # order + 2: the tile is filled with outer bits instead of longer runs;
+    # order + 4: tiles are walked by the outer bits' distance from bit 10
     shapes = [(6, 6, 12, 0, 0, 16), (4, 4, 8, 1, 1, 2), (6, 3, 9, 1, 0, 4), (3, 6, 12, 0, 1, 8), (2, 2, 4, 0, 0, 1),
-              (8, 8, 12, 1, 1, 16), (6, 6, 12, 3, 1, 2), (5, 6, 11, 2, 0, 8), (4, 6, 12, 3, 1, 4), (6, 6, 10, 3, 1, 16)]
+              (8, 8, 12, 1, 1, 16), (6, 6, 12, 3, 1, 2), (5, 6, 11, 2, 0, 8), (4, 6, 12, 3, 1, 4), (6, 6, 10, 3, 1, 16),
+              (6, 6, 12, 4, 1, 2), (5, 5, 11, 6, 1, 8)]
     try:
         for p in perms:
             ref = oracle.permute(x, dims, p)
